@@ -1,0 +1,210 @@
+"""Mapping throughput on one MI355X: reads/s of the fused FP8 + FP32-rescore
+path at the 100 Mbp configuration, with top-20 recall (err = 0.05) beside
+the FP16 path, the fused kernel's roofline and the CPU golden path timed on
+the host cores.
+
+Workload (one step = one batch):
+  reference  100 Mbp, 4 records x 25 Mbp, i.i.d. bases, GC 0.41 (synthetic)
+  index      k = 20,000, kstep = 2,000, s5m1c5 (rdim 2048), fwd + rc windows
+             -> N = 99,928 e4m3 columns resident in HBM
+  reads      65,536 reads of 20 kbp sampled :even from the N-free windows,
+             substitution rate drawn per read from {0, 0.02, 0.05, 0.10},
+             packed words resident in HBM before timing
+  step       encode -> fused e4m3 GEMM + top-32 -> FP32 rescore -> top-20
+
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--reads M] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "qpg-rotormap_amd"))
+
+K_FRAG, KSTEP, S, M_COPIES, C = 20_000, 2_000, 5, 1, 5
+REC_LEN, N_REC, GC = 25_000_000, 4, 0.41
+ERRS = (0.0, 0.02, 0.05, 0.10)
+PEAK_FP8_TFLOPS = 5000.0   # MI355X dense FP8 MFMA
+PEAK_FP16_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
+PEAK_HBM_GBS = 8000.0
+
+
+def lscpu_summary():
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"),
+            "threads_per_core": info.get("Thread(s) per core"), "nproc": os.cpu_count()}
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", "gemm_topk_fp8_100mbp_pmc.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--reads", type=int, default=65_536)
+    ap.add_argument("--fp16-steps", type=int, default=1)
+    ap.add_argument("--cpu-reads", type=int, default=0, help="0 = size the CPU sample to ~15 s")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    import rotormap_amd as R
+    from rotormap_amd import _lib as L
+
+    t0 = time.time()
+    log = lambda *a: print(f"[bench {time.time() - t0:7.1f}s]", *a, file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    ref = R.synth_reference([REC_LEN] * N_REC, gc=GC, seed=100)
+    re = R.RopeEncoder(k=K_FRAG, s=S, m=M_COPIES, c=C)
+    log("reference synthesised; building the index on the GPU")
+    idx = R.build_index(ref, re, kstep=KSTEP, batch_size=8192)
+    N = idx.embeds8.shape[0]
+    log(f"index: {N} columns x {idx.embeds8.shape[1]} e4m3")
+    M = args.reads
+    rng = np.random.default_rng(7)
+    errs = np.array(ERRS)[rng.integers(0, len(ERRS), M)]
+    reads = R.sample_reads(ref, M, K_FRAG, err=errs, seed=11, draw="even", model=0)
+    log(f"{M} reads sampled")
+    words = torch.from_numpy(reads.words.view(np.int32).reshape(-1).copy()).to(dev)
+
+    eng8 = R.TopKEngine(idx.embeds8, k=20, kprime=32, mode="fp8_rescore", rows_cap=M)
+    ctx = L.ctx(0)
+
+    def run(eng, steps, warmup, mode_name):
+        for _ in range(warmup):
+            eng.map_words(re, words, M)
+        torch.cuda.synchronize()
+        L.check(L.lib().rm_prof_read(ctx, None, None, None))  # clear
+        L.check(L.lib().rm_prof_enable(ctx, 1))
+        t = time.perf_counter()
+        out = None
+        for _ in range(steps):
+            out = eng.map_words(re, words, M)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        kms = (L.C.c_double(), L.C.c_int64(), L.C.c_double())
+        L.check(L.lib().rm_prof_read(ctx, L.C.byref(kms[0]), L.C.byref(kms[1]), L.C.byref(kms[2])))
+        L.check(L.lib().rm_prof_enable(ctx, 0))
+        log(f"{mode_name}: {steps} steps in {el:.3f}s; fused kernel {kms[0].value / max(1, kms[1].value):.2f} ms/launch")
+        return el, kms[0].value, kms[1].value, kms[2].value, out
+
+    el8, kms8, nl8, flop8, (v8, l8) = run(eng8, args.steps, args.warmup, "fp8+rescore")
+    locs8 = l8.cpu().numpy()
+    sel = np.isclose(errs, 0.05)
+    rec8 = R.count_mapped(locs8[sel], reads.recs[sel], reads.starts[sel], idx.rec_of_col,
+                          idx.starts, K_FRAG)
+    rec8_all = R.count_mapped(locs8, reads.recs, reads.starts, idx.rec_of_col, idx.starts, K_FRAG)
+
+    # plain FP8 (no rescoring) and FP16 on the same batch: recall + reads/s
+    eng8p = R.TopKEngine(eng8.index, k=20, mode="fp8", rows_cap=M)
+    el8p, kms8p, nl8p, _, (_, l8p) = run(eng8p, 1, 0, "fp8")
+    rec8p = R.count_mapped(l8p.cpu().numpy()[sel], reads.recs[sel], reads.starts[sel],
+                           idx.rec_of_col, idx.starts, K_FRAG)
+    eng16 = R.TopKEngine(R.ResidentIndex(idx.embeds8, fp16=True), k=20, mode="fp16", rows_cap=M)
+    el16, kms16, nl16, flop16, (_, l16) = run(eng16, args.fp16_steps, 1, "fp16")
+    rec16 = R.count_mapped(l16.cpu().numpy()[sel], reads.recs[sel], reads.starts[sel],
+                           idx.rec_of_col, idx.starts, K_FRAG)
+
+    value = M * args.steps / el8
+    k_avg_ms = kms8 / max(1, nl8)
+    achieved = (flop8 / max(1, nl8)) / (k_avg_ms * 1e-3) / 1e12
+    result = {
+        "metric": "mapped reads/sec at top-20 recall (err=0.05) FP8+rescore vs FP16; p50/p99 latency",
+        "value": round(value, 1),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el8 / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "e4m3 (fp32 accumulate, fp32 rescore)",
+        "data": "synthetic",
+        "config": {"workload": "100 Mbp synthetic reference (4 x 25 Mbp, GC 0.41), s5m1c5, "
+                               "k=20000 kstep=2000, N=%d e4m3 columns; one batch of %d reads "
+                               "(:even, substitutions at {0,0.02,0.05,0.10}); top-20 after "
+                               "FP32 rescoring of the fused top-32" % (N, M),
+                   "reads_per_step": M, "index_columns": int(N), "rdim": 2048, "ktop": 20,
+                   "kprime": 32, "parallelism": "single GPU"},
+        "recall_err005": {"fp8_rescore": rec8["correct"] / max(1, rec8["total"]),
+                          "fp8": rec8p["correct"] / max(1, rec8p["total"]),
+                          "fp16": rec16["correct"] / max(1, rec16["total"]),
+                          "reads": int(rec8["total"])},
+        "recall_all_errs_fp8_rescore": rec8_all["correct"] / rec8_all["total"],
+        "reads_per_s": {"fp8_rescore": round(value, 1), "fp8": round(M / el8p, 1),
+                        "fp16": round(M * args.fp16_steps / el16, 1)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP8_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP8_TFLOPS, 4),
+                     "traffic": traffic_from_profile(),
+                     "kernel": "gemm_topk_kernel<e4m3>", "kernel_ms": round(k_avg_ms, 3),
+                     "flop_per_launch": flop8 / max(1, nl8)},
+        "roofline_fp16": {"achieved": round((flop16 / max(1, nl16)) / (kms16 / max(1, nl16) * 1e-3) / 1e12, 2),
+                          "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s"},
+    }
+
+    if not args.no_cpu:
+        sys.path.insert(0, ROOT)
+        import oracle.oracle as O
+        cores = min(os.cpu_count() or 1, 16)
+        B = np.ascontiguousarray(idx.embeds8)
+        fn = O.lib().rmo_cpu_map
+        fn.argtypes = [L.C.c_void_p, L.C.c_int64, L.C.c_int, L.C.c_int, L.C.c_int, L.C.c_int,
+                       L.C.c_void_p, L.C.c_int64, L.C.c_int, L.C.c_int, L.C.c_void_p,
+                       L.C.c_void_p, L.C.c_int]
+        fn.restype = None
+
+        def cpu(n_cpu):
+            wsub = np.ascontiguousarray(reads.words[:n_cpu])
+            vals = np.empty((n_cpu, 20), np.float32)
+            locs = np.empty((n_cpu, 20), np.int32)
+            t = time.perf_counter()
+            fn(L.ptr(wsub), n_cpu, K_FRAG, S, M_COPIES, C, L.ptr(B), N, 2048, 20, L.ptr(vals),
+               L.ptr(locs), cores)
+            return time.perf_counter() - t, locs
+
+        t1, _ = cpu(cores)  # one read per thread sizes the ~15 s sample
+        n_cpu = args.cpu_reads or int(min(4096, cores * max(1, round(15.0 / max(t1, 1e-3)))))
+        log(f"cpu baseline: {n_cpu} reads on {cores} threads ({t1:.2f} s per {cores})")
+        cel, locs = cpu(n_cpu)
+        agree = float(np.mean([len(set(locs[i]) & set(locs8[i])) / 20 for i in range(n_cpu)]))
+        result["cpu_baseline"] = {"value": round(n_cpu / cel, 3), "unit": "reads/s",
+                                  "cores": cores, "kind": "port",
+                                  "sample": f"first {n_cpu} reads of the batch, FP64 golden encoder "
+                                            f"+ FP64 dots over all {N} columns + top-20 "
+                                            f"({cel:.1f} s)",
+                                  "top20_overlap_with_gpu": round(agree, 4),
+                                  "host": lscpu_summary()}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,484 @@
+/*
+ * rm_oracle.c -- CPU restatement of RotorMap's mapping hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
+ * product (librotormap_hip.so) never links, loads or calls it.
+ *
+ * Two encoders live here:
+ *   rmo_ref_rope_frag_real  the FP64 golden encoder, restated line by line
+ *                           from RotorMap/encode/reference.jl:67-135
+ *                           (cis(theta*p) in FP64, no recurrence);
+ *   rmo_rope_fixed          the deterministic fixed-point encoder the HIP
+ *                           kernel implements (integer phasor table, int64
+ *                           bin sums, FP64 normalisation in a fixed order).
+ *                           The HIP kernel must match it bit for bit; it is
+ *                           itself pinned to rmo_ref_rope_frag_real by
+ *                           tolerance and to closed forms by the tests.
+ * and the rest of the path: fp16 / OCP e4m3 conversions, the FP64 GEMM of
+ * dequantised operands, the total-order top-k, the fixed-order FP32 rescoring
+ * dot, reverse complement, the s-mer key counts.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math: every
+ * floating operation below is one IEEE-754 correctly rounded operation).
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define RMO_S30 1073741824.0 /* 2^30: fixed-point scale of the phasor table */
+
+/* ------------------------------------------------------------------------ */
+/* Fragment words <-> base codes                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Base j (0-based) sits in word j>>4 at bit 2*(j&15).
+ * RotorMap/encode/reference.jl:59-65 (_frag_codes). */
+void rmo_frag_codes(const uint32_t *words, int k, uint8_t *codes) {
+    for (int j = 0; j < k; ++j)
+        codes[j] = (uint8_t)((words[j >> 4] >> (2 * (j & 15))) & 3u);
+}
+
+/* Codes -> forward 2-bit words, zero-padded tail.
+ * RotorMap/common/harness.jl:112-118 (_ref_forward_pack). */
+void rmo_forward_pack(const uint8_t *codes, int L, uint32_t *words) {
+    int W = (L + 15) / 16;
+    memset(words, 0, sizeof(uint32_t) * (size_t)W);
+    for (int j = 0; j < L; ++j)
+        words[j >> 4] |= (uint32_t)(codes[j] & 3u) << (2 * (j & 15));
+}
+
+/* Byte -> code: A/a=0 C/c=1 G/g=2 T/t=3, every other byte 2 (kept in place).
+ * RotorMap/fasta/reader.jl:21-28 (_LUT3). */
+uint8_t rmo_byte_code(uint8_t b) {
+    switch (b) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 2;
+    }
+}
+
+/* Reverse complement of a k-base fragment, codes -> codes.
+ * RotorMap/index/build.jl:415-422 (_revcomp_words_ref): rc[j] = 3 - codes[k-1-j]. */
+void rmo_revcomp_codes(const uint8_t *codes, int k, uint8_t *rc) {
+    for (int j = 0; j < k; ++j) rc[j] = (uint8_t)(3u - codes[k - 1 - j]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* s-mer field and compact key                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Compact key (0-based csmer) of the window at 0-based p.
+ * RotorMap/encode/reference.jl:73-80: little-endian s-mer field,
+ * lower ^ ((upper * 0x9E3779B1 mod 2^32) & (4^c-1)). */
+static uint32_t rmo_key(const uint8_t *codes, int p, int s, int c) {
+    uint32_t smer = 0;
+    for (int j = 0; j < s; ++j) smer |= (uint32_t)codes[p + j] << (2 * j);
+    uint32_t cbmask = (c >= 16) ? 0xffffffffu : ((1u << (2 * c)) - 1u);
+    uint32_t lower = smer & cbmask;
+    uint32_t upper = (c >= 16) ? 0u : (smer >> (2 * c));
+    return lower ^ ((upper * 0x9E3779B1u) & cbmask);
+}
+
+/* Integer s-mer histogram: counts[csmer] = windows whose compact key is csmer
+ * (before any G-bin zeroing; the bin bookkeeping of reference.jl:72-85). */
+void rmo_smer_counts(const uint8_t *codes, int k, int s, int c, int32_t *counts) {
+    int nbin = 1 << (2 * c);
+    memset(counts, 0, sizeof(int32_t) * (size_t)nbin);
+    int nw = k - s + 1;
+    for (int p = 0; p < nw; ++p) counts[rmo_key(codes, p, s, c)]++;
+}
+
+/* 0-based compact key of the zeroed G-homopolymer bin: 2 * sum_{i<c} 4^i.
+ * RotorMap/encode/reference.jl:86-90. */
+static int rmo_gbin(int c) {
+    int allones = 0;
+    for (int i = 0; i < c; ++i) allones += 1 << (2 * i);
+    return 2 * allones;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 golden encoder (the reference's CPU golden path)                     */
+/* ------------------------------------------------------------------------ */
+
+/* RotorMap/encode/reference.jl:67-135 (_ref_rope_frag + _ref_rope_frag_real).
+ * out[2D] = [Re(b_0..b_{D-1}) ; Im(...)], norms[m].  theta = f * (2pi/k),
+ * f = 1 (m == 1) else 2*idm/(m-1) + 1 (0-based idm); hist[b] += cis(theta*p). */
+void rmo_ref_rope_frag_real(const uint8_t *codes, int k, int s, int m, int c,
+                            int normalize, double *out, double *norms) {
+    int nb4 = 1 << (2 * c);
+    int D = m * nb4;
+    double *hr = (double *)calloc((size_t)D, sizeof(double));
+    double *hi = (double *)calloc((size_t)D, sizeof(double));
+    int nw = k - s + 1;
+    for (int p = 0; p < nw; ++p) {
+        int cs = (int)rmo_key(codes, p, s, c);
+        for (int idm = 0; idm < m; ++idm) {
+            double f = (m == 1) ? 1.0 : 2.0 * idm / (double)(m - 1) + 1.0;
+            double th = f * (2.0 * M_PI / (double)k);
+            double a = th * (double)p;
+            hr[cs * m + idm] += cos(a);
+            hi[cs * m + idm] += sin(a);
+        }
+    }
+    int g = rmo_gbin(c);
+    for (int idm = 0; idm < m; ++idm) { hr[g * m + idm] = 0.0; hi[g * m + idm] = 0.0; }
+    for (int i = 0; i < m; ++i) norms[i] = 0.0;
+    const double eps = 2.220446049250313e-16; /* Julia eps() = eps(Float64) */
+    if (normalize == 0) {
+        double acc = 0.0;
+        for (int b = 0; b < D; ++b) acc += hr[b] * hr[b] + hi[b] * hi[b];
+        double nrm = sqrt(acc) + 1e-16;
+        for (int b = 0; b < D; ++b) { hr[b] /= nrm; hi[b] /= nrm; }
+        norms[0] = nrm;
+    } else {
+        if (normalize == 2 || normalize == 3) {
+            for (int b = 0; b < D; ++b) {
+                double a = sqrt(hr[b] * hr[b] + hi[b] * hi[b]);
+                double d = (normalize == 2) ? (a + eps) : sqrt(a + eps);
+                hr[b] /= d; hi[b] /= d;
+            }
+        }
+        double add = (normalize == 1) ? 1e-7 : eps;
+        for (int idm = 0; idm < m; ++idm) {
+            double acc = 0.0;
+            for (int b = idm; b < D; b += m) acc += hr[b] * hr[b] + hi[b] * hi[b];
+            double nrm = sqrt(acc) + add;
+            for (int b = idm; b < D; b += m) { hr[b] /= nrm; hi[b] /= nrm; }
+            norms[idm] = nrm;
+        }
+    }
+    for (int b = 0; b < D; ++b) { out[b] = hr[b]; out[D + b] = hi[b]; }
+    free(hr); free(hi);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deterministic fixed-point encoder (the HIP kernel's specification)        */
+/* ------------------------------------------------------------------------ */
+
+/* Phasor table: tab[2*(idm*nw + p) + {0,1}] = round(2^30 * {cos,sin}(2pi*q)),
+ * q = frac(f*p/k) formed exactly in integers: f*p/k = (2*idm + m-1)*p /
+ * ((m-1)*k) for m > 1, p/k for m == 1 (the phase of reference.jl:82-83). */
+void rmo_phase_table(int k, int s, int m, int32_t *tab) {
+    int nw = k - s + 1;
+    for (int idm = 0; idm < m; ++idm) {
+        int64_t mul = (m == 1) ? 1 : (int64_t)(2 * idm + m - 1);
+        int64_t den = (m == 1) ? (int64_t)k : (int64_t)(m - 1) * (int64_t)k;
+        for (int p = 0; p < nw; ++p) {
+            int64_t num = (mul * (int64_t)p) % den;
+            double ang = (2.0 * M_PI) * ((double)num / (double)den);
+            tab[2 * ((size_t)idm * nw + p) + 0] = (int32_t)llround(cos(ang) * RMO_S30);
+            tab[2 * ((size_t)idm * nw + p) + 1] = (int32_t)llround(sin(ang) * RMO_S30);
+        }
+    }
+}
+
+/* Sum of e[idx(j)] in the kernel's fixed order: 64 lane partials (lane l adds
+ * items l, l+64, ... sequentially) folded by a halving tree (off = 32..1). */
+static double rmo_tree_sum(const double *e, int D, int m, int idm) {
+    double part[64];
+    for (int l = 0; l < 64; ++l) {
+        double acc = 0.0;
+        for (int j = l; idm + (int64_t)j * m < D; j += 64) acc = acc + e[idm + j * m];
+        part[l] = acc;
+    }
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int l = 0; l < off; ++l) part[l] = part[l] + part[l + off];
+    return part[0];
+}
+
+/* out_f64[2D] holds the normalised values before storage rounding; the
+ * storage conversions (fp32, fp16, e4m3) are applied by the caller.
+ * acc_out (optional, 2D int64, interleaved re/im) returns the bin sums. */
+void rmo_rope_fixed(const uint8_t *codes, int k, int s, int m, int c, int normalize,
+                    const int32_t *tab, double *out, double *norms, int64_t *acc_out) {
+    int nb4 = 1 << (2 * c);
+    int D = m * nb4;
+    int nw = k - s + 1;
+    int64_t *ar = (int64_t *)calloc((size_t)D, sizeof(int64_t));
+    int64_t *ai = (int64_t *)calloc((size_t)D, sizeof(int64_t));
+    for (int p = 0; p < nw; ++p) {
+        int cs = (int)rmo_key(codes, p, s, c);
+        for (int idm = 0; idm < m; ++idm) {
+            ar[cs * m + idm] += tab[2 * ((size_t)idm * nw + p) + 0];
+            ai[cs * m + idm] += tab[2 * ((size_t)idm * nw + p) + 1];
+        }
+    }
+    int g = rmo_gbin(c);
+    for (int idm = 0; idm < m; ++idm) { ar[g * m + idm] = 0; ai[g * m + idm] = 0; }
+    if (acc_out)
+        for (int b = 0; b < D; ++b) { acc_out[2 * b] = ar[b]; acc_out[2 * b + 1] = ai[b]; }
+    double *x = (double *)malloc(sizeof(double) * (size_t)D);
+    double *y = (double *)malloc(sizeof(double) * (size_t)D);
+    double *e = (double *)malloc(sizeof(double) * (size_t)D);
+    for (int b = 0; b < D; ++b) {
+        x[b] = (double)ar[b] * (1.0 / RMO_S30);
+        y[b] = (double)ai[b] * (1.0 / RMO_S30);
+    }
+    const double eps = 2.220446049250313e-16;
+    if (normalize == 2 || normalize == 3) {
+        for (int b = 0; b < D; ++b) {
+            double t1 = x[b] * x[b], t2 = y[b] * y[b];
+            double a = sqrt(t1 + t2);
+            double d = (normalize == 2) ? (a + eps) : sqrt(a + eps);
+            x[b] = x[b] / d; y[b] = y[b] / d;
+        }
+    }
+    for (int b = 0; b < D; ++b) { double t1 = x[b] * x[b], t2 = y[b] * y[b]; e[b] = t1 + t2; }
+    double cp[64];
+    for (int idm = 0; idm < m; ++idm) cp[idm] = rmo_tree_sum(e, D, m, idm);
+    if (normalize == 0) {
+        double tot = cp[0];
+        for (int idm = 1; idm < m; ++idm) tot = tot + cp[idm];
+        double nrm = sqrt(tot) + 1e-16;
+        norms[0] = nrm;
+        for (int idm = 1; idm < m; ++idm) norms[idm] = cp[idm];
+        for (int b = 0; b < D; ++b) { out[b] = x[b] / nrm; out[D + b] = y[b] / nrm; }
+    } else {
+        double add = (normalize == 1) ? 1e-7 : eps;
+        double nr[64];
+        for (int idm = 0; idm < m; ++idm) { nr[idm] = sqrt(cp[idm]) + add; norms[idm] = nr[idm]; }
+        for (int b = 0; b < D; ++b) { out[b] = x[b] / nr[b % m]; out[D + b] = y[b] / nr[b % m]; }
+    }
+    free(ar); free(ai); free(x); free(y); free(e);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Storage formats: IEEE binary16 (RNE) and OCP FP8 E4M3FN (RNE, satfinite)   */
+/* ------------------------------------------------------------------------ */
+
+static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* fp32 -> binary16 bits, round to nearest even (IEEE 754-2008, 5.3). */
+uint16_t rmo_f32_to_f16(float f) {
+    uint32_t u = f2u(f);
+    uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+    uint32_t a = u & 0x7fffffffu;
+    if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);      /* NaN */
+    if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);     /* >= 65520 -> inf */
+    if (a < 0x38800000u) {                                       /* below 2^-14: subnormal */
+        /* value / 2^-24 rounded to nearest even integer */
+        float v = u2f(a) * 16777216.0f;                          /* exact (power of two) */
+        float r = rintf(v);
+        return (uint16_t)(sign | (uint16_t)r);
+    }
+    uint32_t e = (a >> 23) - 127 + 15;
+    uint32_t mant = a & 0x7fffffu;
+    uint32_t h = (e << 10) | (mant >> 13);
+    uint32_t rem = mant & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h += 1;   /* carry may bump exponent */
+    return (uint16_t)(sign | h);
+}
+
+float rmo_f16_to_f32(uint16_t h) {
+    uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu, mant = h & 0x3ffu;
+    if (e == 0) {
+        float v = (float)mant * (1.0f / 16777216.0f);
+        return (h & 0x8000u) ? -v : v;
+    }
+    if (e == 31) return u2f(sign | 0x7f800000u | (mant << 13));
+    return u2f(sign | ((e - 15 + 127) << 23) | (mant << 13));
+}
+
+/* fp32 -> e4m3fn: bias 7, max 448 (S.1111.110), NaN S.1111.111, no inf.
+ * Round to nearest even; |x| > 448 saturates to 448 (the satfinite rule the
+ * query side uses, RotorMap/gemm/fp8_ptx.jl:165).  Written from the OCP
+ * 8-bit floating point specification (v1.0, section 4.1). */
+uint8_t rmo_f32_to_e4m3(float f) {
+    uint32_t u = f2u(f);
+    uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+    float a = fabsf(f);
+    if (a != a) return (uint8_t)(sign | 0x7fu);
+    if (a > 448.0f) return (uint8_t)(sign | 0x7eu);
+    if (a < 0.015625f) {                       /* below 2^-6: multiples of 2^-9 */
+        float q = rintf(a * 512.0f);            /* exact scaling; 0..8 */
+        return (uint8_t)(sign | (uint8_t)q);    /* q == 8 is the min normal code */
+    }
+    int e = (int)((f2u(a) >> 23) & 0xffu) - 127; /* -6 .. 8 */
+    float mant = a / ldexpf(1.0f, e);           /* [1, 2), exact */
+    float q = rintf((mant - 1.0f) * 8.0f);      /* 0 .. 8 */
+    int qi = (int)q;
+    if (qi == 8) { e += 1; qi = 0; }
+    int code = ((e + 7) << 3) | qi;
+    if (code > 0x7e) code = 0x7e;
+    return (uint8_t)(sign | (uint8_t)code);
+}
+
+float rmo_e4m3_to_f32(uint8_t b) {
+    float sgn = (b & 0x80u) ? -1.0f : 1.0f;
+    int e = (b >> 3) & 0xf, m = b & 7;
+    if (e == 15 && m == 7) return u2f((b & 0x80u) ? 0xffc00000u : 0x7fc00000u);
+    if (e == 0) return sgn * ldexpf((float)m, -9);
+    return sgn * ldexpf(1.0f + (float)m / 8.0f, e - 7);
+}
+
+/* The storage chain both sides of the GEMM share (RotorMap/index/build.jl:730,
+ * RotorMap/search/engine_fp8.jl:365-368): e4m3(fp32(fp16(fp32(x)))). */
+uint8_t rmo_f64_to_e4m3_chain(double v) {
+    float f = (float)v;
+    return rmo_f32_to_e4m3(rmo_f16_to_f32(rmo_f32_to_f16(f)));
+}
+
+/* ------------------------------------------------------------------------ */
+/* GEMM (FP64 of dequantised operands) and top-k                             */
+/* ------------------------------------------------------------------------ */
+
+/* C[r*N + j] = sum_k deq(Q[r*K + k]) * deq(B[j*K + k]) in FP64: the score
+ * model of RotorMap/search/engine_fp8.jl:873-876 (Bh' * hrefq). */
+void rmo_gemm_e4m3_f64(const uint8_t *Q, const uint8_t *B, int64_t M, int64_t N, int64_t K,
+                       double *C) {
+    float lut[256];
+    for (int i = 0; i < 256; ++i) lut[i] = rmo_e4m3_to_f32((uint8_t)i);
+    double *qr = (double *)malloc(sizeof(double) * (size_t)K);
+    for (int64_t r = 0; r < M; ++r) {
+        for (int64_t k = 0; k < K; ++k) qr[k] = lut[Q[r * K + k]];
+        for (int64_t j = 0; j < N; ++j) {
+            const uint8_t *bj = B + j * K;
+            double acc = 0.0;
+            for (int64_t k = 0; k < K; ++k) acc += qr[k] * (double)lut[bj[k]];
+            C[r * N + j] = acc;
+        }
+    }
+    free(qr);
+}
+
+/* Same for binary16 operands (the FP16 path). */
+void rmo_gemm_f16_f64(const uint16_t *Q, const uint16_t *B, int64_t M, int64_t N, int64_t K,
+                      double *C) {
+    double *qr = (double *)malloc(sizeof(double) * (size_t)K);
+    double *br = (double *)malloc(sizeof(double) * (size_t)K);
+    for (int64_t r = 0; r < M; ++r) {
+        for (int64_t k = 0; k < K; ++k) qr[k] = rmo_f16_to_f32(Q[r * K + k]);
+        for (int64_t j = 0; j < N; ++j) {
+            for (int64_t k = 0; k < K; ++k) br[k] = rmo_f16_to_f32(B[j * K + k]);
+            double acc = 0.0;
+            for (int64_t k = 0; k < K; ++k) acc += qr[k] * br[k];
+            C[r * N + j] = acc;
+        }
+    }
+    free(qr); free(br);
+}
+
+/* (value desc, column asc): a strict total order on (v, col) pairs. */
+static int rmo_beats(float va, int32_t ca, float vb, int32_t cb) {
+    return va > vb || (va == vb && ca < cb);
+}
+
+/* Per row of a row-major (M x N) fp32 score matrix: the top-k under the total
+ * order, as 1-based columns (0 = empty, value -inf), the output convention of
+ * RotorMap/gemm/topk_kernels.jl:56-60,141-160. */
+void rmo_topk_rows_f32(const float *C, int64_t M, int64_t N, int64_t ldc, int k,
+                       float *vals, int32_t *locs) {
+    for (int64_t r = 0; r < M; ++r) {
+        float *v = vals + r * k;
+        int32_t *l = locs + r * k;
+        for (int j = 0; j < k; ++j) { v[j] = -INFINITY; l[j] = 0; }
+        int fill = 0;
+        for (int64_t j = 0; j < N; ++j) {
+            float x = C[r * ldc + j];
+            int32_t cj = (int32_t)(j + 1);
+            if (fill == k && !rmo_beats(x, cj, v[k - 1], l[k - 1])) continue;
+            int pos = (fill < k) ? fill : k - 1;
+            while (pos > 0 && rmo_beats(x, cj, v[pos - 1], l[pos - 1])) {
+                v[pos] = v[pos - 1]; l[pos] = l[pos - 1]; --pos;
+            }
+            v[pos] = x; l[pos] = cj;
+            if (fill < k) ++fill;
+        }
+    }
+}
+
+/* The rescoring dot in its fixed FP32 order: 64 lanes, lane l owns elements
+ * [l*C, (l+1)*C) (C = ceil(K/64)), each a sequential sum of q*deq(b) products
+ * (multiply, then add: two roundings), folded by an xor butterfly (off = 32..1,
+ * s[l] = s[l] + s[l^off]).  Returns the value every lane ends with. */
+float rmo_rescore_dot(const float *q, const uint8_t *b, int K) {
+    float s[64], t[64];
+    int C = (K + 63) / 64;
+    for (int l = 0; l < 64; ++l) {
+        float acc = 0.0f;
+        for (int j = l * C; j < (l + 1) * C && j < K; ++j) {
+            float p = q[j] * rmo_e4m3_to_f32(b[j]);
+            acc = acc + p;
+        }
+        s[l] = acc;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        for (int l = 0; l < 64; ++l) t[l] = s[l] + s[l ^ off];
+        memcpy(s, t, sizeof(s));
+    }
+    return s[0];
+}
+
+/* Rescore a candidate list: scores[j] = rmo_rescore_dot(q, B[col_j - 1]),
+ * then order by (score desc, col asc) and keep the first ktop.  Empty
+ * candidates (col 0) stay empty. */
+void rmo_rescore_rank(const float *q, const uint8_t *B, int K, const int32_t *cand, int nc,
+                      int ktop, float *vals, int32_t *locs) {
+    float sc[1024];
+    int32_t cc[1024];
+    int n = 0;
+    for (int j = 0; j < nc && n < 1024; ++j) {
+        if (cand[j] <= 0) continue;
+        sc[n] = rmo_rescore_dot(q, B + (int64_t)(cand[j] - 1) * K, K);
+        cc[n] = cand[j];
+        ++n;
+    }
+    for (int j = 0; j < ktop; ++j) { vals[j] = -INFINITY; locs[j] = 0; }
+    for (int i = 0; i < n; ++i) {
+        int rank = 0; /* candidates are distinct columns: ranks are distinct */
+        for (int j = 0; j < n; ++j) rank += rmo_beats(sc[j], cc[j], sc[i], cc[i]);
+        if (rank < ktop) { vals[rank] = sc[i]; locs[rank] = cc[i]; }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* The reference's CPU golden mapping path, threaded (cpu_baseline only)      */
+/* ------------------------------------------------------------------------ */
+
+/* Per read: FP64 golden encoder (reference.jl:67-135) -> the query storage
+ * chain e4m3(fp32(fp16(.))) (engine_fp8.jl:365-368) -> FP64 dot products
+ * against every dequantised e4m3 column (engine_fp8.jl:873-876) -> top-ktop
+ * by (score desc, column asc) (engine_fp8.jl:875).  OpenMP over reads. */
+void rmo_cpu_map(const uint32_t *words, int64_t n, int k, int s, int m, int c,
+                 const uint8_t *B, int64_t N, int K, int ktop, float *vals, int32_t *locs,
+                 int nthreads) {
+    float lut[256];
+    for (int i = 0; i < 256; ++i) lut[i] = rmo_e4m3_to_f32((uint8_t)i);
+    const int W = (k + 15) / 16;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+    for (int64_t r = 0; r < n; ++r) {
+        uint8_t *codes = (uint8_t *)malloc((size_t)k);
+        double *emb = (double *)malloc(sizeof(double) * (size_t)K);
+        double *q = (double *)malloc(sizeof(double) * (size_t)K);
+        double nr[64];
+        rmo_frag_codes(words + r * W, k, codes);
+        rmo_ref_rope_frag_real(codes, k, s, m, c, 0, emb, nr);
+        for (int j = 0; j < K; ++j) q[j] = lut[rmo_f64_to_e4m3_chain(emb[j])];
+        float *v = vals + r * ktop;
+        int32_t *l = locs + r * ktop;
+        for (int j = 0; j < ktop; ++j) { v[j] = -INFINITY; l[j] = 0; }
+        int fill = 0;
+        for (int64_t col = 0; col < N; ++col) {
+            const uint8_t *b = B + col * K;
+            double acc = 0.0;
+            for (int j = 0; j < K; ++j) acc += q[j] * (double)lut[b[j]];
+            float x = (float)acc;
+            int32_t cj = (int32_t)(col + 1);
+            if (fill == ktop && !rmo_beats(x, cj, v[ktop - 1], l[ktop - 1])) continue;
+            int pos = (fill < ktop) ? fill : ktop - 1;
+            while (pos > 0 && rmo_beats(x, cj, v[pos - 1], l[pos - 1])) {
+                v[pos] = v[pos - 1]; l[pos] = l[pos - 1]; --pos;
+            }
+            v[pos] = x; l[pos] = cj;
+            if (fill < ktop) ++fill;
+        }
+        free(codes); free(emb); free(q);
+    }
+}

@@ -1,0 +1,595 @@
+// capi.cpp -- the C ABI of librotormap_hip.so: argument checking, context /
+// index lifetime, workspaces, and the host orchestration of the kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "rm_internal.h"
+
+#define RM_VERSION "rotormap_hip 0.1.0 (gfx950)"
+
+namespace rm {
+
+static thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+void clear_error() { g_err.clear(); }
+
+// NULL is the device's default (null) stream, as in the HIP runtime API; the
+// context's own stream only carries the library's internal uploads.
+hipStream_t pick_stream(rm_ctx *, void *stream) { return (hipStream_t)stream; }
+
+int ws_get(rm_ctx *ctx, const char *name, size_t bytes, void **out) {
+    rm_buf &b = ctx->ws[name];
+    if (b.n < bytes) {
+        if (b.p) {
+            RM_HIP(hipDeviceSynchronize());
+            RM_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.n = 0;
+        }
+        size_t want = std::max(bytes, (size_t)256);
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e != hipSuccess) {
+            b.p = nullptr;
+            return fail(RM_ERR_NOMEM, "workspace '%s': hipMalloc(%zu) failed: %s", name, want,
+                        hipGetErrorString(e));
+        }
+        b.n = want;
+    }
+    *out = b.p;
+    return RM_OK;
+}
+
+int table_get(rm_ctx *ctx, int k, int s, int m, const int32_t **out) {
+    auto key = std::make_tuple(k, s, m);
+    auto it = ctx->tables.find(key);
+    if (it != ctx->tables.end()) {
+        *out = it->second;
+        return RM_OK;
+    }
+    size_t cnt = (size_t)2 * m * (k - s + 1);
+    std::vector<int32_t> h(cnt);
+    build_phase_table(k, s, m, h.data());
+    int32_t *d = nullptr;
+    RM_HIP(hipMalloc(&d, cnt * 4));
+    RM_HIP(hipMemcpy(d, h.data(), cnt * 4, hipMemcpyHostToDevice));
+    ctx->tables[key] = d;
+    ctx->table_bytes += cnt * 4;
+    *out = d;
+    return RM_OK;
+}
+
+// Column splits for M query rows over N index columns: enough (row tile,
+// split) items for two resident workgroups per CU, never more splits than
+// column tiles; the autotuned plan overrides per M bucket.
+int default_splits(rm_ctx *ctx, int64_t M, int64_t N) {
+    int64_t R = cdiv(M, kBR), tiles = cdiv(N, kBI);
+    int64_t want = cdiv(2 * (int64_t)ctx->num_cus, R);
+    int64_t S = std::max<int64_t>(1, std::min<int64_t>(want, tiles));
+    return (int)std::min<int64_t>(S, 256);
+}
+
+static int bucket_of(int64_t M) {
+    int b = 0;
+    while (((int64_t)1 << b) < M) ++b;
+    return b;  // ceil(log2 M)
+}
+
+static int splits_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode) {
+    auto it = ctx->plan.find({mode == RM_MODE_FP16 ? 1 : 0, bucket_of(M)});
+    int64_t tiles = cdiv(idx->N, kBI);
+    if (it != ctx->plan.end()) return (int)std::max<int64_t>(1, std::min<int64_t>(it->second, tiles));
+    return default_splits(ctx, M, idx->N);
+}
+
+int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
+              const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
+              float *vals, int32_t *locs, int64_t ldo, int S, hipStream_t st) {
+    const int KP = (mode == RM_MODE_FP8_RESCORE) ? kprime : ktop;
+    const int64_t K = idx->rdim;
+    const int esz = elem == 0 ? 1 : 2;
+    const uint8_t *db = elem == 0 ? idx->b8 : (const uint8_t *)idx->b16;
+    RM_CHECK(db, RM_ERR_STATE, "index has no %s copy (create it with RM_INDEX_FP16)",
+             elem == 0 ? "e4m3" : "fp16");
+    if (M == 0) return RM_OK;
+    const int64_t tiles = cdiv(idx->N, kBI);
+    S = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)S, tiles, (int64_t)256}));
+    void *pv = nullptr, *pc = nullptr;
+    size_t pbytes = (size_t)M * S * KP * 4;
+    int rc;
+    if ((rc = ws_get(ctx, "part_v", pbytes, &pv))) return rc;
+    if ((rc = ws_get(ctx, "part_c", pbytes, &pc))) return rc;
+    GemmArgs a{};
+    a.db = db;
+    a.ldb = K * esz;
+    a.q = (const uint8_t *)Q;
+    a.ldq = ldq_bytes;
+    a.M = M;
+    a.N = idx->N;
+    a.Kb = (int)(K * esz);
+    a.S = S;
+    a.R = cdiv(M, kBR);
+    a.tiles = tiles;
+    a.col_base = 0;
+    a.part_v = (float *)pv;
+    a.part_c = (int32_t *)pc;
+    a.KP = KP;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->prof) {
+        RM_HIP(hipEventCreate(&e0));
+        RM_HIP(hipEventCreate(&e1));
+        RM_HIP(hipEventRecord(e0, st));
+    }
+    if ((rc = launch_gemm_topk(a, elem, st))) return rc;
+    if (ctx->prof) {
+        RM_HIP(hipEventRecord(e1, st));
+        ctx->prof_ev.push_back({e0, e1});
+        ctx->prof_flop += 2.0 * (double)M * (double)K * (double)idx->N;
+    }
+    return launch_merge((const float *)pv, (const int32_t *)pc, S, KP, M, ktop, mode, Q32, ldq32,
+                        idx->b8, K, (int)K, vals, locs, ldo, nullptr, st);
+}
+
+static int check_encoder(int k, int s, int m, int c, int normalize) {
+    RM_CHECK(c >= 1 && c <= s && s <= 16, RM_ERR_ARG,
+             "encoder needs 1 <= c <= s <= 16 (got s = %d, c = %d)", s, c);
+    RM_CHECK(s <= k, RM_ERR_ARG, "k = %d < s = %d: no s-mer windows", k, s);
+    RM_CHECK(m >= 1 && m <= 64, RM_ERR_ARG, "m must be in 1..64 (got %d)", m);
+    RM_CHECK(2LL * m * (1LL << (2 * c)) <= 16384, RM_ERR_ARG, "2*m*4^c = %lld exceeds 16384",
+             2LL * m * (1LL << (2 * c)));
+    RM_CHECK(normalize >= 0 && normalize <= 3, RM_ERR_ARG, "normalize must be 0..3 (got %d)",
+             normalize);
+    return RM_OK;
+}
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace rm
+
+using namespace rm;
+
+extern "C" {
+
+const char *rm_version(void) { return RM_VERSION; }
+const char *rm_last_error(void) { return g_err.c_str(); }
+
+int rm_ctx_create(int device, rm_ctx **out) {
+    RM_CHECK(out, RM_ERR_ARG, "out is NULL");
+    int ndev = 0;
+    RM_HIP(hipGetDeviceCount(&ndev));
+    RM_CHECK(device >= 0 && device < ndev, RM_ERR_ARG, "device %d out of range (%d devices)",
+             device, ndev);
+    RM_HIP(hipSetDevice(device));
+    rm_ctx *c = new rm_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(RM_ERR_HIP, "hipStreamCreateWithFlags failed");
+    }
+    *out = c;
+    return RM_OK;
+}
+
+int rm_ctx_destroy(rm_ctx *ctx) {
+    if (!ctx) return RM_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->tables) hipFree(kv.second);
+    for (auto &kv : ctx->ws)
+        if (kv.second.p) hipFree(kv.second.p);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RM_OK;
+}
+
+int rm_sync(rm_ctx *ctx, void *stream) {
+    RM_CHECK(ctx, RM_ERR_ARG, "ctx is NULL");
+    RM_HIP(hipStreamSynchronize(pick_stream(ctx, stream)));
+    return RM_OK;
+}
+
+int64_t rm_ctx_bytes(rm_ctx *ctx) {
+    if (!ctx) return 0;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    int64_t s = (int64_t)ctx->table_bytes;
+    for (auto &kv : ctx->ws) s += (int64_t)kv.second.n;
+    return s;
+}
+
+int rm_phase_table(int k, int s, int m, int32_t *host_out) {
+    RM_CHECK(host_out, RM_ERR_ARG, "host_out is NULL");
+    RM_CHECK(k >= 1 && s >= 1 && s <= k && m >= 1 && m <= 64, RM_ERR_ARG, "bad (k, s, m)");
+    build_phase_table(k, s, m, host_out);
+    return RM_OK;
+}
+
+int rm_encode_batch(rm_ctx *ctx, const uint32_t *words, int64_t n, int k, int s, int m, int c,
+                    int normalize, void *out1, int dtype1, int64_t sf1, int64_t sd1, void *out2,
+                    int dtype2, int64_t sf2, int64_t sd2, float *norms, int32_t *counts,
+                    void *stream) {
+    RM_CHECK(ctx, RM_ERR_ARG, "ctx is NULL");
+    int rc;
+    if ((rc = check_encoder(k, s, m, c, normalize))) return rc;
+    RM_CHECK(n >= 0, RM_ERR_ARG, "n < 0");
+    if (n == 0) return RM_OK;
+    RM_CHECK(words && out1, RM_ERR_ARG, "words / out1 is NULL");
+    RM_CHECK(dtype1 >= RM_F32 && dtype1 <= RM_E4M3, RM_ERR_ARG, "bad dtype1 %d", dtype1);
+    RM_CHECK(!out2 || (dtype2 >= RM_F32 && dtype2 <= RM_E4M3), RM_ERR_ARG, "bad dtype2 %d", dtype2);
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    RM_HIP(hipSetDevice(ctx->device));
+    const int32_t *tab;
+    if ((rc = table_get(ctx, k, s, m, &tab))) return rc;
+    EncOut o1{out1, dtype1, sf1, sd1}, o2{out2, dtype2, sf2, sd2};
+    return launch_encode(words, n, k, s, m, c, normalize, tab, o1, o2, norms, counts,
+                         pick_stream(ctx, stream));
+}
+
+int rm_revcomp_words(rm_ctx *ctx, uint32_t *rc, const uint32_t *words, int64_t n, int k,
+                     void *stream) {
+    RM_CHECK(ctx && rc && words, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(k >= 1 && n >= 0, RM_ERR_ARG, "bad k / n");
+    RM_CHECK(rc != words, RM_ERR_ARG, "in-place reverse complement is not supported");
+    return launch_revcomp(rc, words, n, k, pick_stream(ctx, stream));
+}
+
+int rm_extract_windows(rm_ctx *ctx, uint32_t *out, const uint32_t *rec_words,
+                       const int64_t *starts, int64_t n, int k, void *stream) {
+    RM_CHECK(ctx && out && rec_words && starts, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(k >= 1 && n >= 0, RM_ERR_ARG, "bad k / n");
+    return launch_extract(out, rec_words, starts, n, k, pick_stream(ctx, stream));
+}
+
+int rm_quantize_e4m3(rm_ctx *ctx, uint8_t *dst, const void *src, int src_dtype, int64_t n,
+                     void *stream) {
+    RM_CHECK(ctx && dst && src, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(src_dtype == RM_F32 || src_dtype == RM_F16, RM_ERR_ARG, "src_dtype must be F32 or F16");
+    return launch_quantize(dst, src, src_dtype, n, pick_stream(ctx, stream));
+}
+
+int rm_dequantize_e4m3(rm_ctx *ctx, float *dst, const uint8_t *src, int64_t n, void *stream) {
+    RM_CHECK(ctx && dst && src, RM_ERR_ARG, "NULL argument");
+    return launch_dequantize(dst, src, n, pick_stream(ctx, stream));
+}
+
+int rm_gemm_chunk(rm_ctx *ctx, void *C, int c_dtype, int64_t ldc, const void *A,
+                  int64_t a_stride_row, int64_t a_stride_k, const void *B, int64_t ldb, int ab_dtype,
+                  int64_t M, int64_t wi, int64_t K, void *stream) {
+    RM_CHECK(ctx && C && A && B, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(c_dtype == RM_F32 || c_dtype == RM_F16, RM_ERR_ARG, "c_dtype must be F32 or F16");
+    RM_CHECK(ab_dtype == RM_E4M3 || ab_dtype == RM_F16, RM_ERR_ARG, "ab_dtype must be E4M3 or F16");
+    RM_CHECK(M >= 0 && wi >= 0 && K > 0, RM_ERR_ARG, "bad M / wi / K");
+    const int esz = ab_dtype == RM_E4M3 ? 1 : 2;
+    RM_CHECK((K * esz) % kBK == 0, RM_ERR_SHAPE, "K*elemsize = %lld must be a multiple of 64",
+             (long long)(K * esz));
+    RM_CHECK(ldb >= K && (ldb * esz) % 16 == 0 && aligned16(B), RM_ERR_SHAPE,
+             "B needs ldb >= K, 16-byte aligned columns");
+    RM_CHECK(ldc >= M, RM_ERR_SHAPE, "ldc < M");
+    if (M == 0 || wi == 0) return RM_OK;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    hipStream_t st = pick_stream(ctx, stream);
+    const uint8_t *q = (const uint8_t *)A;
+    int64_t ldq = a_stride_row * esz;
+    if (!(a_stride_k == 1 && (ldq % 16) == 0 && aligned16(A))) {
+        void *t;
+        int rc;
+        if ((rc = ws_get(ctx, "a_rowmajor", (size_t)(M * K * esz), &t))) return rc;
+        if ((rc = launch_transpose_u8((uint8_t *)t, (const uint8_t *)A, M, K, a_stride_row,
+                                      a_stride_k, esz, st)))
+            return rc;
+        q = (const uint8_t *)t;
+        ldq = K * esz;
+    }
+    GemmArgs a{};
+    a.db = (const uint8_t *)B;
+    a.ldb = ldb * esz;
+    a.q = q;
+    a.ldq = ldq;
+    a.M = M;
+    a.N = wi;
+    a.Kb = (int)(K * esz);
+    a.R = cdiv(M, kBR);
+    a.tiles = cdiv(wi, kBI);
+    a.S = (int)a.tiles;
+    a.C = C;
+    a.c_dtype = c_dtype;
+    a.ldc = ldc;
+    return launch_gemm_store(a, ab_dtype == RM_E4M3 ? 0 : 1, st);
+}
+
+int rm_rowtopk_merge(rm_ctx *ctx, float *D_val, int32_t *D_loc, int64_t ldd, const void *C,
+                     int c_dtype, int64_t ldc, int64_t M, int64_t wi, int64_t offset, int k,
+                     void *stream) {
+    RM_CHECK(ctx && D_val && D_loc && C, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(c_dtype >= RM_F32 && c_dtype <= RM_E4M3, RM_ERR_ARG, "bad c_dtype");
+    RM_CHECK(k >= 1 && k <= 64, RM_ERR_ARG, "k must be 1..64");
+    RM_CHECK(wi >= 1, RM_ERR_ARG, "chunk must be non-empty");
+    RM_CHECK(ldd >= M && ldc >= M, RM_ERR_SHAPE, "leading dimensions < M");
+    return launch_rowtopk(D_val, D_loc, ldd, C, c_dtype, ldc, M, wi, offset, k,
+                          pick_stream(ctx, stream));
+}
+
+int rm_index_create(rm_ctx *ctx, const uint8_t *B, int64_t rdim, int64_t N, int flags,
+                    rm_index **out) {
+    RM_CHECK(ctx && B && out, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(rdim > 0 && rdim % 64 == 0, RM_ERR_SHAPE, "rdim = %lld must be a positive multiple of 64",
+             (long long)rdim);
+    RM_CHECK(N >= 1 && N < ((int64_t)1 << 31) - 1, RM_ERR_SHAPE, "N = %lld out of range",
+             (long long)N);
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    RM_HIP(hipSetDevice(ctx->device));
+    rm_index *ix = new rm_index();
+    ix->ctx = ctx;
+    ix->rdim = rdim;
+    ix->N = N;
+    size_t bytes = (size_t)rdim * N;
+    if (hipMalloc(&ix->b8, bytes) != hipSuccess) {
+        delete ix;
+        return fail(RM_ERR_NOMEM, "index: hipMalloc(%zu) failed", bytes);
+    }
+    hipError_t e = hipMemcpyAsync(ix->b8, B, bytes,
+                                  (flags & RM_INDEX_HOST) ? hipMemcpyHostToDevice
+                                                          : hipMemcpyDeviceToDevice,
+                                  ctx->stream);
+    if (e == hipSuccess && (flags & RM_INDEX_FP16)) {
+        if (hipMalloc(&ix->b16, bytes * 2) != hipSuccess) {
+            hipFree(ix->b8);
+            delete ix;
+            return fail(RM_ERR_NOMEM, "index fp16 copy: hipMalloc(%zu) failed", bytes * 2);
+        }
+        int rc = launch_widen_f16(ix->b16, ix->b8, (int64_t)bytes, ctx->stream);
+        if (rc) { hipFree(ix->b8); hipFree(ix->b16); delete ix; return rc; }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        hipFree(ix->b8);
+        if (ix->b16) hipFree(ix->b16);
+        delete ix;
+        return fail(RM_ERR_HIP, "index upload failed: %s", hipGetErrorString(e));
+    }
+    *out = ix;
+    return RM_OK;
+}
+
+int rm_index_destroy(rm_index *idx) {
+    if (!idx) return RM_OK;
+    hipSetDevice(idx->ctx->device);
+    hipDeviceSynchronize();
+    if (idx->b8) hipFree(idx->b8);
+    if (idx->b16) hipFree(idx->b16);
+    delete idx;
+    return RM_OK;
+}
+
+int64_t rm_index_cols(const rm_index *idx) { return idx ? idx->N : 0; }
+int64_t rm_index_rdim(const rm_index *idx) { return idx ? idx->rdim : 0; }
+const uint8_t *rm_index_e4m3(const rm_index *idx) { return idx ? idx->b8 : nullptr; }
+const void *rm_index_f16(const rm_index *idx) { return idx ? idx->b16 : nullptr; }
+
+static int check_query(const rm_index *idx, int ktop, int kprime, int mode) {
+    RM_CHECK(mode == RM_MODE_FP8 || mode == RM_MODE_FP8_RESCORE || mode == RM_MODE_FP16,
+             RM_ERR_ARG, "bad mode %d", mode);
+    RM_CHECK(ktop >= 1 && ktop <= kMaxKP, RM_ERR_ARG, "ktop must be 1..%d (got %d)", kMaxKP, ktop);
+    if (mode == RM_MODE_FP8_RESCORE)
+        RM_CHECK(kprime >= ktop && kprime <= kMaxKP, RM_ERR_ARG,
+                 "kprime must be in ktop..%d (got %d)", kMaxKP, kprime);
+    RM_CHECK(ktop <= idx->N, RM_ERR_ARG, "ktop = %d > N = %lld", ktop, (long long)idx->N);
+    if (mode == RM_MODE_FP16) RM_CHECK(idx->b16, RM_ERR_STATE, "index has no fp16 copy");
+    return RM_OK;
+}
+
+int rm_topk_batch(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq, const float *Q32,
+                  int64_t ldq32, int64_t M, int ktop, int kprime, int mode, float *vals,
+                  int32_t *locs, int64_t ldo, void *stream) {
+    RM_CHECK(ctx && idx && vals && locs, RM_ERR_ARG, "NULL argument");
+    int rc;
+    if ((rc = check_query(idx, ktop, kprime, mode))) return rc;
+    RM_CHECK(M >= 0 && ldo >= M, RM_ERR_SHAPE, "need M >= 0 and ldo >= M");
+    if (M == 0) return RM_OK;
+    RM_CHECK(Q, RM_ERR_ARG, "Q is NULL");
+    const int elem = mode == RM_MODE_FP16 ? 1 : 0;
+    const int64_t ldq_b = ldq * (elem ? 2 : 1);
+    RM_CHECK(ldq >= idx->rdim && ldq_b % 16 == 0 && aligned16(Q), RM_ERR_SHAPE,
+             "Q rows must be >= rdim elements, 16-byte aligned");
+    if (mode == RM_MODE_FP8_RESCORE) RM_CHECK(Q32 && ldq32 >= idx->rdim, RM_ERR_ARG, "Q32 required");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    RM_HIP(hipSetDevice(ctx->device));
+    return topk_core(ctx, idx, Q, ldq_b, elem, Q32, ldq32, M, ktop, kprime, mode, vals, locs, ldo,
+                     splits_for(ctx, idx, M, mode), pick_stream(ctx, stream));
+}
+
+// encode n fragments straight into the query layouts the GEMM reads
+static int encode_queries(rm_ctx *ctx, const uint32_t *words, int64_t n, int k, int s, int m,
+                          int c, int mode, int64_t rdim, float *norms, void **q, float **q32,
+                          hipStream_t st) {
+    int rc;
+    const int32_t *tab;
+    if ((rc = table_get(ctx, k, s, m, &tab))) return rc;
+    const int elem = mode == RM_MODE_FP16 ? 1 : 0;
+    if ((rc = ws_get(ctx, "q", (size_t)n * rdim * (elem ? 2 : 1), q))) return rc;
+    EncOut o1{*q, elem ? RM_F16 : RM_E4M3, rdim, 1}, o2{nullptr, 0, 0, 0};
+    *q32 = nullptr;
+    if (mode == RM_MODE_FP8_RESCORE) {
+        void *p;
+        if ((rc = ws_get(ctx, "q32", (size_t)n * rdim * 4, &p))) return rc;
+        *q32 = (float *)p;
+        o2 = EncOut{p, RM_F32, rdim, 1};
+    }
+    return launch_encode(words, n, k, s, m, c, 0, tab, o1, o2, norms, nullptr, st);
+}
+
+int rm_map_batch(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_t n, int k,
+                 int s, int m, int c, int ktop, int kprime, int mode, float *vals, int32_t *locs,
+                 int64_t ldo, float *norms, void *stream) {
+    RM_CHECK(ctx && idx && vals && locs, RM_ERR_ARG, "NULL argument");
+    int rc;
+    if ((rc = check_encoder(k, s, m, c, 0))) return rc;
+    if ((rc = check_query(idx, ktop, kprime, mode))) return rc;
+    const int64_t rdim = 2LL * m * (1LL << (2 * c));
+    RM_CHECK(rdim == idx->rdim, RM_ERR_SHAPE, "encoder dim 2*m*4^c = %lld != index rdim %lld",
+             (long long)rdim, (long long)idx->rdim);
+    RM_CHECK(n >= 0 && ldo >= n, RM_ERR_SHAPE, "need n >= 0 and ldo >= n");
+    if (n == 0) return RM_OK;
+    RM_CHECK(words, RM_ERR_ARG, "words is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    RM_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = pick_stream(ctx, stream);
+    void *q;
+    float *q32;
+    if ((rc = encode_queries(ctx, words, n, k, s, m, c, mode, rdim, norms, &q, &q32, st))) return rc;
+    const int elem = mode == RM_MODE_FP16 ? 1 : 0;
+    return topk_core(ctx, idx, q, rdim * (elem ? 2 : 1), elem, q32, rdim, n, ktop, kprime, mode,
+                     vals, locs, ldo, splits_for(ctx, idx, n, mode), st);
+}
+
+int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, const int64_t *offs,
+                   int nb, int k, int s, int m, int c, int ktop, int kprime, int mode, float *vals,
+                   int32_t *locs, int64_t ldo, void *stream) {
+    RM_CHECK(ctx && idx && offs && vals && locs, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(nb >= 0, RM_ERR_ARG, "nb < 0");
+    if (nb == 0) return RM_OK;
+    RM_CHECK(offs[0] == 0, RM_ERR_ARG, "offs[0] must be 0");
+    for (int b = 0; b < nb; ++b)
+        RM_CHECK(offs[b + 1] >= offs[b], RM_ERR_ARG, "offs must be non-decreasing");
+    // rows are independent: the concatenation is one launch whose (row tile,
+    // split) work list spans every batch -- no per-batch launch, no per-batch
+    // tail; the split count follows the total row count
+    return rm_map_batch(ctx, idx, words, offs[nb], k, s, m, c, ktop, kprime, mode, vals, locs, ldo,
+                        nullptr, stream);
+}
+
+int rm_plan_splits(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode) {
+    if (!ctx || !idx || M <= 0) return 0;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return splits_for(ctx, idx, M, mode);
+}
+
+int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int splits) {
+    RM_CHECK(ctx && M >= 1 && splits >= 0 && splits <= 256, RM_ERR_ARG, "bad plan entry");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    auto key = std::make_pair(mode == RM_MODE_FP16 ? 1 : 0, bucket_of(M));
+    if (splits == 0) ctx->plan.erase(key);
+    else ctx->plan[key] = splits;
+    return RM_OK;
+}
+
+int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, int nbuckets,
+                int mode, int32_t *best_splits, float *best_ms) {
+    RM_CHECK(ctx && idx && bucket_rows && nbuckets >= 0, RM_ERR_ARG, "NULL argument");
+    int rc;
+    if ((rc = check_query(idx, 20, 32, mode))) return rc;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    RM_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int elem = mode == RM_MODE_FP16 ? 1 : 0;
+    const int64_t rdim = idx->rdim;
+    const int64_t tiles = cdiv(idx->N, kBI);
+    hipEvent_t e0, e1;
+    RM_HIP(hipEventCreate(&e0));
+    RM_HIP(hipEventCreate(&e1));
+    for (int b = 0; b < nbuckets; ++b) {
+        const int64_t M = bucket_rows[b];
+        RM_CHECK(M >= 1, RM_ERR_ARG, "bucket rows must be >= 1");
+        void *q, *q32 = nullptr, *vo, *lo;
+        if ((rc = ws_get(ctx, "tune_q", (size_t)M * rdim * (elem ? 2 : 1), &q))) return rc;
+        if (mode == RM_MODE_FP8_RESCORE && (rc = ws_get(ctx, "tune_q32", (size_t)M * rdim * 4, &q32)))
+            return rc;
+        if ((rc = ws_get(ctx, "tune_v", (size_t)M * 20 * 4, &vo))) return rc;
+        if ((rc = ws_get(ctx, "tune_l", (size_t)M * 20 * 4, &lo))) return rc;
+        RM_HIP(hipMemsetAsync(q, 0x1c, (size_t)M * rdim * (elem ? 2 : 1), st));
+        if (q32) RM_HIP(hipMemsetAsync(q32, 0, (size_t)M * rdim * 4, st));
+        const int64_t R = cdiv(M, kBR);
+        std::vector<int> cands;
+        for (int S = 1; S <= 256 && S <= tiles; S *= 2) {
+            if (R * S > 64LL * ctx->num_cus && S > 1) break;
+            cands.push_back(S);
+        }
+        int bestS = cands.empty() ? 1 : cands[0];
+        float bestT = 1e30f;
+        for (int S : cands) {
+            if ((rc = topk_core(ctx, idx, q, rdim * (elem ? 2 : 1), elem, (float *)q32, rdim, M, 20,
+                                32, mode, (float *)vo, (int32_t *)lo, M, S, st)))
+                return rc;  // warm-up
+            RM_HIP(hipEventRecord(e0, st));
+            const int reps = 2;
+            for (int r = 0; r < reps; ++r)
+                if ((rc = topk_core(ctx, idx, q, rdim * (elem ? 2 : 1), elem, (float *)q32, rdim, M,
+                                    20, 32, mode, (float *)vo, (int32_t *)lo, M, S, st)))
+                    return rc;
+            RM_HIP(hipEventRecord(e1, st));
+            RM_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            RM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            if (ms < bestT) { bestT = ms; bestS = S; }
+        }
+        ctx->plan[{elem, bucket_of(M)}] = bestS;
+        if (best_splits) best_splits[b] = bestS;
+        if (best_ms) best_ms[b] = bestT;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return RM_OK;
+}
+
+int rm_prof_enable(rm_ctx *ctx, int on) {
+    RM_CHECK(ctx, RM_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->prof = on != 0;
+    return RM_OK;
+}
+
+int rm_prof_read(rm_ctx *ctx, double *kernel_ms, int64_t *launches, double *flop) {
+    RM_CHECK(ctx, RM_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    double ms = 0;
+    for (auto &p : ctx->prof_ev) {
+        RM_HIP(hipEventSynchronize(p.second));
+        float t = 0;
+        RM_HIP(hipEventElapsedTime(&t, p.first, p.second));
+        ms += t;
+        hipEventDestroy(p.first);
+        hipEventDestroy(p.second);
+    }
+    if (kernel_ms) *kernel_ms = ms;
+    if (launches) *launches = (int64_t)ctx->prof_ev.size();
+    if (flop) *flop = ctx->prof_flop;
+    ctx->prof_ev.clear();
+    ctx->prof_flop = 0;
+    return RM_OK;
+}
+
+int64_t rm_batch_bytes(int64_t M, int k, int rdim, int ktop, int kprime, int mode) {
+    const int KP = (mode == RM_MODE_FP8_RESCORE) ? kprime : ktop;
+    const int64_t W = (k + 15) / 16;
+    int64_t b = M * W * 4;                                  // packed words
+    b += M * rdim * (mode == RM_MODE_FP16 ? 2 : 1);         // encoded queries
+    if (mode == RM_MODE_FP8_RESCORE) b += M * rdim * 4;     // fp32 rescoring copy
+    b += M * 256LL * KP * 8;                                // split partials (<= 256 splits)
+    b += M * (int64_t)ktop * 8;                             // vals + locs
+    return b;
+}
+
+int64_t rm_device_free_bytes(rm_ctx *ctx) {
+    if (!ctx) return -1;
+    size_t fr = 0, tot = 0;
+    hipSetDevice(ctx->device);
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return -1;
+    return (int64_t)fr;
+}
+
+}  // extern "C"

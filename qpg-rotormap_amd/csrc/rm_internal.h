@@ -1,0 +1,150 @@
+// rm_internal.h -- shared host-side declarations of librotormap_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/rotormap_hip.h"
+
+namespace rm {
+
+// thread-local last error + status helper
+int fail(int code, const char *fmt, ...);
+void clear_error();
+
+#define RM_HIP(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) return rm::fail(RM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+#define RM_CHECK(cond, code, ...)                       \
+    do {                                                \
+        if (!(cond)) return rm::fail(code, __VA_ARGS__); \
+    } while (0)
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- fused GEMM + top-k tile geometry (shared by host planning and kernels)
+constexpr int kBI = 256;       // index columns per tile
+constexpr int kBR = 128;       // reads per tile
+constexpr int kBK = 64;        // K bytes per LDS stage
+constexpr int kRow = 80;       // LDS row pitch in bytes (64 + 16 pad)
+constexpr int kCap = 32;       // candidate slots per read per merge round
+constexpr int kThreads = 256;  // 4 waves
+constexpr int kMaxKP = 64;     // largest top-k' list
+
+// LDS bytes of the fused kernel for a list length kp
+inline size_t topk_lds_bytes(int kp) {
+    size_t stage = 2 * (size_t)(kBI + kBR) * kRow;
+    size_t lists = (size_t)kBR * (kp + 1) * 8;
+    size_t cands = (size_t)kBR * (kCap + 1) * 8;
+    size_t misc = (size_t)kBR * 8 + 64;
+    return stage + lists + cands + misc;
+}
+inline size_t store_lds_bytes() { return 2 * (size_t)(kBI + kBR) * kRow; }
+
+struct GemmArgs {
+    const uint8_t *db;  // index rows (K contiguous bytes each), row pitch ldb bytes
+    int64_t ldb;
+    const uint8_t *q;   // query rows, row pitch ldq bytes
+    int64_t ldq;
+    int64_t M;          // query rows
+    int64_t N;          // index rows covered by this launch
+    int Kb;             // K in bytes
+    int S;              // column splits
+    int64_t R;          // row tiles = ceil(M / kBR)
+    int64_t tiles;      // column tiles = ceil(N / kBI)
+    int64_t col_base;   // global column id of db row 0
+    // top-k epilogue
+    float *part_v;
+    int32_t *part_c;
+    int KP;
+    // store-C epilogue
+    void *C;
+    int c_dtype;
+    int64_t ldc;
+    // persistent work list (uneven batches): items of (row tile, split)
+    const int32_t *work;  // NULL: grid mapping; else 3 ints per item {rowtile, split, S}
+    int64_t nwork;
+    int *work_counter;
+};
+
+// launchers (gemm_topk.hip)
+int launch_gemm_topk(const GemmArgs &a, int elem, hipStream_t st);
+int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st);
+int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int ktop,
+                 int mode, const float *q32, int64_t ldq32, const uint8_t *db, int64_t ldb,
+                 int K, float *vals, int32_t *locs, int64_t ldo, const int64_t *row_split_off,
+                 hipStream_t st);
+int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dtype, int64_t ldc,
+                   int64_t M, int64_t wi, int64_t offset, int k, hipStream_t st);
+int launch_transpose_u8(uint8_t *dst, const uint8_t *src, int64_t rows, int64_t cols,
+                        int64_t s_row, int64_t s_col, int esize, hipStream_t st);
+
+// launchers (encode.hip)
+struct EncOut {
+    void *p;
+    int dtype;
+    int64_t sf, sd;
+};
+int launch_encode(const uint32_t *words, int64_t n, int k, int s, int m, int c, int normalize,
+                  const int32_t *table, EncOut o1, EncOut o2, float *norms, int32_t *counts,
+                  hipStream_t st);
+int launch_revcomp(uint32_t *rc, const uint32_t *w, int64_t n, int k, hipStream_t st);
+int launch_extract(uint32_t *out, const uint32_t *rec, const int64_t *starts, int64_t n, int k,
+                   hipStream_t st);
+int launch_quantize(uint8_t *dst, const void *src, int src_dtype, int64_t n, hipStream_t st);
+int launch_dequantize(float *dst, const uint8_t *src, int64_t n, hipStream_t st);
+int launch_widen_f16(uint16_t *dst, const uint8_t *src, int64_t n, hipStream_t st);
+
+void build_phase_table(int k, int s, int m, int32_t *tab);
+
+}  // namespace rm
+
+// growable device workspace slots
+struct rm_buf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct rm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    std::recursive_mutex mu;
+    std::map<std::tuple<int, int, int>, int32_t *> tables;  // (k, s, m) -> device phasor table
+    size_t table_bytes = 0;
+    std::map<std::string, rm_buf> ws;
+    // autotuned splits: (mode, M bucket) -> S
+    std::map<std::pair<int, int>, int> plan;
+    // kernel timing (rm_prof_enable): event pairs around fused launches
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
+    double prof_flop = 0;
+};
+
+struct rm_index {
+    rm_ctx *ctx = nullptr;
+    uint8_t *b8 = nullptr;    // N x rdim e4m3 (column j contiguous)
+    uint16_t *b16 = nullptr;  // N x rdim fp16 widening (optional)
+    int64_t rdim = 0, N = 0;
+};
+
+namespace rm {
+// workspace slot of at least `bytes` (grows; contents undefined)
+int ws_get(rm_ctx *ctx, const char *name, size_t bytes, void **out);
+int table_get(rm_ctx *ctx, int k, int s, int m, const int32_t **out);
+hipStream_t pick_stream(rm_ctx *ctx, void *stream);
+int default_splits(rm_ctx *ctx, int64_t M, int64_t N);
+// the whole fused query path on already-encoded queries
+int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
+              const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
+              float *vals, int32_t *locs, int64_t ldo, int S, hipStream_t st);
+}  // namespace rm

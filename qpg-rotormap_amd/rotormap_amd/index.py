@@ -1,0 +1,157 @@
+"""Index layer: build the e4m3 window database on the GPU, save / load it.
+
+Windows follow RotorMap/index/build.jl:291-324 (_each_ref_window!): per
+record, k-windows at 0-based offsets 0, kstep, 2*kstep, ... fully inside the
+record (nwin = (L - k) // kstep + 1; shorter records contribute nothing).
+Columns per build batch are [fwd nb | rc nb] with shared heads/starts and
+strand 0/1 (:649-657); batch_size=None is one batch over all windows (the
+tiny harness layout, RotorMap/common/harness.jl:177-245).  Each column is
+e4m3(fp32(fp16(encoder output))) (:730), written by the encoder kernel
+straight into the resident layout.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .encode import RopeEncoder, encode_frag_real_batch, rdim as enc_rdim, revcomp_frag_words
+
+FP8_FORMAT = "indexreal.fp8.v1"
+
+
+@dataclass
+class IndexData:
+    """The fields of an indexreal.fp8.v1 index (RotorMap/index/build.jl:755-768);
+    embeds8 is stored (N, rdim) row-major = the reference's (rdim, N)
+    column-major matrix."""
+    embeds8: np.ndarray
+    norms: np.ndarray     # (N, m) = the reference's (m, N) column-major
+    heads: list
+    starts: np.ndarray    # 1-based window starts
+    strand: np.ndarray    # 0 fwd, 1 rc
+    k: int
+    kstep: int
+    s: int
+    m: int
+    c: int
+    normalize: int = 0
+    n_frag: int = 0
+    source: str = ""
+    format: str = FP8_FORMAT
+    rec_of_col: np.ndarray = None   # 0-based record index per column
+    rec_names: list = field(default_factory=list)
+
+    @property
+    def encoder(self):
+        return RopeEncoder(k=self.k, s=self.s, m=self.m, c=self.c)
+
+
+def window_table(ref, k, kstep):
+    """(rec index, 0-based start) of every window in file order."""
+    recs, starts = [], []
+    for j, r in enumerate(ref.records):
+        if r.length < k:
+            continue
+        nwin = (r.length - k) // kstep + 1
+        recs.append(np.full(nwin, j, dtype=np.int64))
+        starts.append(np.arange(nwin, dtype=np.int64) * kstep)
+    if not recs:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    return np.concatenate(recs), np.concatenate(starts)
+
+
+def build_index(ref, re, kstep=None, batch_size=None, device=0, normalize=0, chunk=1 << 16):
+    """Encode every window of `ref` (both strands) on the GPU into e4m3 columns."""
+    import torch
+    k = re.k
+    kstep = kstep or max(k // 10, 1)
+    rd = enc_rdim(re)
+    W = (k + 15) // 16
+    wrec, wst = window_table(ref, k, kstep)
+    nwin = len(wrec)
+    if nwin == 0:
+        raise L.RotorMapArgumentError(L.RM_ERR_ARG, "no k-window fits any record")
+    bs = nwin if batch_size is None else int(batch_size)
+    # column of window g: batch b = g // bs holds columns [2*b*bs, ...): fwd
+    # rows first (nb of them), then the rc rows
+    g = np.arange(nwin, dtype=np.int64)
+    b = g // bs
+    nb = np.minimum(bs, nwin - b * bs)
+    col_f = 2 * b * bs + (g - b * bs)
+    col_r = col_f + nb
+    N = 2 * nwin
+    dev = f"cuda:{device}"
+    emb = torch.empty(N, rd, dtype=torch.uint8, device=dev)
+    norms = torch.empty(N, re.m, dtype=torch.float32, device=dev)
+    recw = [torch.from_numpy(r.words.view(np.int32)).to(dev) for r in ref.records]
+    for lo in range(0, nwin, chunk):
+        hi = min(nwin, lo + chunk)
+        n = hi - lo
+        fw = torch.empty(n * W, dtype=torch.int32, device=dev)
+        for j in np.unique(wrec[lo:hi]):
+            sel = np.where(wrec[lo:hi] == j)[0]
+            st = torch.from_numpy(wst[lo:hi][sel]).to(dev)
+            tmp = torch.empty(len(sel) * W, dtype=torch.int32, device=dev)
+            L.check(L.lib().rm_extract_windows(L.ctx(device), L.ptr(tmp), L.ptr(recw[j]), L.ptr(st),
+                                               len(sel), k, L.stream_of(device)))
+            idx = torch.from_numpy(sel).to(dev)
+            fw.view(n, W)[idx] = tmp.view(len(sel), W)
+        rc = torch.empty_like(fw)
+        revcomp_frag_words(rc, fw, k, n)
+        for words, cols in ((fw, col_f[lo:hi]), (rc, col_r[lo:hi])):
+            e = torch.empty(n, rd, dtype=torch.uint8, device=dev)
+            nn = torch.empty(n, re.m, dtype=torch.float32, device=dev)
+            encode_frag_real_batch(e, nn.t(), re, words, normalize)
+            ci = torch.from_numpy(cols).to(dev)
+            emb[ci] = e
+            norms[ci] = nn
+    torch.cuda.synchronize(device)
+    names = [r.name for r in ref.records]
+    rec_of_col = np.empty(N, dtype=np.int64)
+    starts = np.empty(N, dtype=np.int64)
+    strand = np.empty(N, dtype=np.uint8)
+    rec_of_col[col_f] = wrec
+    rec_of_col[col_r] = wrec
+    starts[col_f] = wst + 1
+    starts[col_r] = wst + 1
+    strand[col_f] = 0
+    strand[col_r] = 1
+    heads = [ref.records[j].header for j in rec_of_col.tolist()]
+    return IndexData(emb.cpu().numpy(), norms.cpu().numpy(), heads, starts, strand, k, kstep,
+                     re.s, re.m, re.c, normalize, nwin, "", FP8_FORMAT, rec_of_col, names)
+
+
+def save_index(d, path):
+    """npz container of the indexreal.fp8.v1 fields (no pickled objects)."""
+    uniq = sorted(set(d.heads), key=d.heads.index)
+    hid = np.array([uniq.index(h) for h in d.heads], dtype=np.int32) if len(uniq) < 64 else \
+        np.unique(np.array(d.heads), return_inverse=True)[1].astype(np.int32)
+    if len(uniq) >= 64:
+        uniq = list(np.unique(np.array(d.heads)))
+    np.savez(path, embeds8=d.embeds8, norms=d.norms, head_ids=hid,
+             head_table=np.array(uniq, dtype=np.str_), starts=d.starts, strand=d.strand,
+             meta=np.array([d.k, d.kstep, d.s, d.m, d.c, d.normalize, d.n_frag], dtype=np.int64),
+             format=np.array(d.format), source=np.array(d.source))
+
+
+def load_index(path):
+    """Inverse of save_index; asserts normalize = 0 like RotorMap/index/load.jl:57-59."""
+    z = np.load(path, allow_pickle=False)
+    k, kstep, s, m, c, normalize, n_frag = z["meta"].tolist()
+    if str(z["format"]) != FP8_FORMAT:
+        raise L.RotorMapArgumentError(L.RM_ERR_ARG, f"not an {FP8_FORMAT} index: {path}")
+    if normalize != 0:
+        raise L.RotorMapArgumentError(L.RM_ERR_ARG, "the mapping flow assumes a normalize = 0 index")
+    table = [str(x) for x in z["head_table"]]
+    heads = [table[i] for i in z["head_ids"].tolist()]
+    names = []
+    for h in heads:
+        nm = h.split()[0][1:]
+        if nm not in names:
+            names.append(nm)
+    rec_of_col = np.array([names.index(h.split()[0][1:]) for h in heads], dtype=np.int64)
+    rd = 2 * m * 4 ** c
+    e8 = z["embeds8"]
+    assert e8.shape == (2 * n_frag, rd), "embeds8 shape mismatch"
+    return IndexData(e8, z["norms"], heads, z["starts"], z["strand"], k, kstep, s, m, c, normalize,
+                     n_frag, str(z["source"]), FP8_FORMAT, rec_of_col, names)

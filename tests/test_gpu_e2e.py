@@ -1,0 +1,91 @@
+"""End-to-end mapping on the GPU at the 1 Mbp plain configuration: two
+records (600 kbp + 400 kbp), s8m4c4, k = 20,000, kstep = 2,000 -> N = 964
+columns; 1000 exact reads must all map (RotorMap/experiments/e2e_fp8.jl:
+141-163) and >= 90 % of 200 reads with 5 % substitutions must map in the
+top 20 (RotorMap/selftest.jl:202,218-220)."""
+import numpy as np
+import pytest
+
+import oracle.oracle as O
+import rotormap_amd as R
+
+pytestmark = pytest.mark.gpu
+
+K, KSTEP = 20_000, 2_000
+
+
+@pytest.fixture(scope="module")
+def tiny(torch_gpu):
+    ref = R.synth_reference([600_000, 400_000], gc=0.5, seed=2024)
+    re = R.RopeEncoder(k=K, s=8, m=4, c=4)
+    idx = R.build_index(ref, re, kstep=KSTEP)
+    assert idx.embeds8.shape == (964, 2048)
+    exact = R.sample_reads(ref, 1000, K, err=0.0, seed=42, draw="random")
+    mut = R.sample_reads(ref, 200, K, err=0.05, seed=43, draw="random", model=0)
+    return ref, re, idx, exact, mut
+
+
+def run(torch, idx, re, reads, mode, kprime=32):
+    eng = R.TopKEngine(idx.embeds8, k=20, kprime=kprime, mode=mode)
+    n = reads.words.shape[0]
+    dw = torch.from_numpy(reads.words.view(np.int32).reshape(-1).copy()).cuda()
+    v, l = eng.map_words(re, dw, n)
+    torch.cuda.synchronize()
+    return v.cpu().numpy(), l.cpu().numpy()
+
+
+def score(idx, reads, locs):
+    return R.count_mapped(locs, reads.recs, reads.starts, idx.rec_of_col, idx.starts, K)
+
+
+@pytest.mark.parametrize("mode", ["fp8_rescore", "fp8", "fp16"])
+def test_tiny_e2e_recall(torch_gpu, tiny, mode):
+    ref, re, idx, exact, mut = tiny
+    _, l0 = run(torch_gpu, idx, re, exact, mode)
+    r0 = score(idx, exact, l0)
+    assert r0["correct"] == 1000, r0
+    _, l1 = run(torch_gpu, idx, re, mut, mode)
+    r1 = score(idx, mut, l1)
+    assert r1["correct"] >= 180, r1
+
+
+def test_tiny_index_matches_oracle(torch_gpu, tiny):
+    """Index columns = e4m3(fp16(fp32(fixed-point encoder))) of fwd / rc windows."""
+    ref, re, idx, _, _ = tiny
+    rng = np.random.default_rng(0)
+    nwin = idx.n_frag
+    for col in rng.choice(2 * nwin, size=8, replace=False):
+        j, st, strand = idx.rec_of_col[col], idx.starts[col], idx.strand[col]
+        codes = R.unpack_words(ref.records[j].words[:-1], ref.records[j].length)[st - 1:st - 1 + K]
+        if strand == 1:
+            codes = O.revcomp_codes(codes)
+        out, _ = O.rope_fixed(codes, K, 8, 4, 4, 0)
+        np.testing.assert_array_equal(idx.embeds8[col], O.store_chain(out, "e4m3"))
+
+
+def test_tiny_scores_vs_fp64(torch_gpu, tiny):
+    """Top-20 values vs an FP64 model of the quantised pipeline (the
+    reference's _check_topk_stream, atol 5e-3: RotorMap/search/engine_fp8.jl:869-883)."""
+    ref, re, idx, _, mut = tiny
+    v, l = run(torch_gpu, idx, re, mut, "fp8")
+    deq = O.e4m3_table()[idx.embeds8].astype(np.float64)
+    for r in range(0, 200, 25):
+        codes = O.frag_codes(mut.words[r], K)
+        h, _ = O.rope_fixed(codes, K, 8, 4, 4, 0)
+        q = O.e4m3_table()[O.store_chain(h, "e4m3")].astype(np.float64)
+        scores = deq @ q
+        refv = np.sort(scores)[::-1][:20]
+        np.testing.assert_allclose(v[r], refv, atol=5e-3, rtol=0)
+        assert np.all(np.abs(scores[l[r] - 1] - v[r]) <= 5e-3)
+
+
+def test_alone_equals_in_mix(torch_gpu, tiny):
+    """Per-row results do not depend on the batch (deterministic kernels)."""
+    ref, re, idx, exact, mut = tiny
+    for mode in ("fp8_rescore", "fp16"):
+        v_all, l_all = run(torch_gpu, idx, re, mut, mode)
+        sub = R.ReadSet(mut.words[5:9].copy(), mut.recs[5:9], mut.starts[5:9], mut.err[5:9], K,
+                        mut.names)
+        v_one, l_one = run(torch_gpu, idx, re, sub, mode)
+        np.testing.assert_array_equal(v_one, v_all[5:9])
+        np.testing.assert_array_equal(l_one, l_all[5:9])

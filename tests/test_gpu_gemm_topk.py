@@ -1,0 +1,203 @@
+"""GEMM + fused top-k parity on the GPU.
+
+The GEMM is checked against the FP64 product of the stored operands within
+the reference's tolerances (RotorMap/gemm/fp8_ptx.jl:248-268).  The fused
+top-k is checked BIT-EXACTLY: the fused kernel's accumulators are the same
+MFMA sequence as the C-writing chunk kernel, so the oracle's total-order
+top-k over the GPU's own C must equal the fused result value for value and
+column for column, for every split count.  Shapes are the reference's top-k
+test list (RotorMap/gemm/topk_kernels.jl:521-523)."""
+import numpy as np
+import pytest
+
+import oracle.oracle as O
+import rotormap_amd as R
+from rotormap_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(64, 100), (257, 128), (1000, 333), (1, 4096), (511, 2048), (2048, 2 ** 14), (33, 37),
+          (64, 25), (64, 20)]
+
+
+def rand_e4m3(shape, scale, seed):
+    rng = np.random.default_rng(seed)
+    x = (rng.standard_normal(shape) * scale).astype(np.float32)
+    return O.f32_to_e4m3(x).reshape(shape)
+
+
+def unit_rows_e4m3(n, K, seed):
+    """Unit-norm rows quantised like the pipeline (fp32 -> fp16 -> e4m3)."""
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, K))
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    return O.store_chain(x.ravel(), "e4m3").reshape(n, K), x.astype(np.float32)
+
+
+def gemm_c(torch, Q, B, out="f32"):
+    M, K = Q.shape
+    N = B.shape[0]
+    dq = torch.from_numpy(Q).cuda()
+    db = torch.from_numpy(B).cuda()
+    C = torch.zeros(N, M, dtype=torch.float32 if out == "f32" else torch.float16, device="cuda").t()
+    R.fp8_gemm_hip(C, dq, db, N)
+    torch.cuda.synchronize()
+    return C.cpu().numpy().astype(np.float32)
+
+
+def test_gemm_chunk_fp8_vs_fp64(torch_gpu):
+    torch = torch_gpu
+    Q = rand_e4m3((256, 256), 0.05, 1)
+    B = rand_e4m3((256, 256), 0.02, 2)
+    ref = O.gemm_e4m3_f64(Q, B)
+    C32 = gemm_c(torch, Q, B)
+    rel = np.linalg.norm(C32 - ref) / np.linalg.norm(ref)
+    assert rel < 1e-6, rel
+    C16 = gemm_c(torch, Q, B, out="f16")
+    assert np.linalg.norm(C16 - ref) / np.linalg.norm(ref) < 5e-3
+
+
+def test_gemm_chunk_ragged_and_layouts(torch_gpu):
+    torch = torch_gpu
+    M, N, K = 300, 700, 2048
+    Q = rand_e4m3((M, K), 0.03, 3)
+    B = rand_e4m3((N, K), 0.03, 4)
+    ref = O.gemm_e4m3_f64(Q, B)
+    C = gemm_c(torch, Q, B)
+    assert np.max(np.abs(C - ref)) < 1e-5
+    # the reference's column-major A (a8: element (r, k) at r + k*M) gives the same bits
+    dq_col = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda().t()
+    db = torch.from_numpy(B).cuda()
+    C2 = torch.zeros(N, M, dtype=torch.float32, device="cuda").t()
+    R.fp8_gemm_hip(C2, dq_col, db, N)
+    # chunked B columns (the chunk loop's base-pointer offsets) == whole, bitwise
+    C3 = torch.zeros(N, M, dtype=torch.float32, device="cuda").t()
+    for lo in range(0, N, 256):
+        wi = min(N, lo + 256) - lo
+        R.fp8_gemm_hip(C3[:, lo:lo + wi], torch.from_numpy(Q).cuda(), db, wi, b_col0=lo)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(C2.cpu().numpy(), C)
+    np.testing.assert_array_equal(C3.cpu().numpy(), C)
+
+
+def test_gemm_chunk_fp16_vs_fp64(torch_gpu):
+    torch = torch_gpu
+    rng = np.random.default_rng(9)
+    Q = (rng.standard_normal((200, 512)) * 0.05).astype(np.float16)
+    B = (rng.standard_normal((333, 512)) * 0.05).astype(np.float16)
+    ref = O.gemm_f16_f64(Q, B)
+    C = torch.zeros(333, 200, dtype=torch.float32, device="cuda").t()
+    R.fp8_gemm_hip(C, torch.from_numpy(Q).cuda(), torch.from_numpy(B).cuda(), 333)
+    torch.cuda.synchronize()
+    got = C.cpu().numpy()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-6
+
+
+def fused(torch, eng, Q, Q32=None, splits=None):
+    M = Q.shape[0]
+    mode = eng.mode
+    if splits is not None:
+        L.check(R.lib().rm_plan_set(L.ctx(0), M, mode, splits))
+    dq = torch.from_numpy(Q).cuda()
+    d32 = torch.from_numpy(Q32).cuda() if Q32 is not None else None
+    v, l = eng.batch_gemm_topk(dq, d32)
+    torch.cuda.synchronize()
+    if splits is not None:
+        L.check(R.lib().rm_plan_set(L.ctx(0), M, mode, 0))
+    return v.cpu().numpy(), l.cpu().numpy()
+
+
+@pytest.mark.parametrize("M,N", SHAPES)
+def test_fused_topk_bitexact(torch_gpu, M, N):
+    torch = torch_gpu
+    K = 512
+    k = 20
+    Q = rand_e4m3((M, K), 0.05, M)
+    B = rand_e4m3((N, K), 0.05, N)
+    C = gemm_c(torch, Q, B)
+    ev, el = O.topk_rows(C, k)
+    eng = R.TopKEngine(B, k=k, mode="fp8")
+    tiles = -(-N // 256)
+    for S in sorted({1, 2, 3, tiles}):
+        if S > tiles:
+            continue
+        v, l = fused(torch, eng, Q, splits=S)
+        np.testing.assert_array_equal(l, el, err_msg=f"locs, splits={S}")
+        np.testing.assert_array_equal(v, ev, err_msg=f"vals, splits={S}")
+
+
+def test_fused_topk_ties_and_k64(torch_gpu):
+    torch = torch_gpu
+    # duplicate columns force exact ties: order must be (value desc, column asc)
+    K, M = 512, 130
+    base = rand_e4m3((40, K), 0.05, 11)
+    B = np.concatenate([base] * 30)  # 1200 columns, each value 30 times
+    Q = rand_e4m3((M, K), 0.05, 12)
+    C = gemm_c(torch, Q, B)
+    for k in (20, 64):
+        ev, el = O.topk_rows(C, k)
+        eng = R.TopKEngine(B, k=k, mode="fp8")
+        for S in (1, 4):
+            v, l = fused(torch, eng, Q, splits=S)
+            np.testing.assert_array_equal(l, el)
+            np.testing.assert_array_equal(v, ev)
+
+
+def test_fused_topk_fp16_bitexact(torch_gpu):
+    torch = torch_gpu
+    rng = np.random.default_rng(21)
+    M, N, K = 300, 1500, 512
+    Q = (rng.standard_normal((M, K)) * 0.05).astype(np.float16)
+    B8 = rand_e4m3((N, K), 0.05, 22)
+    B16 = O.e4m3_table()[B8].astype(np.float16)  # exact widening
+    C = torch.zeros(N, M, dtype=torch.float32, device="cuda").t()
+    R.fp8_gemm_hip(C, torch.from_numpy(Q).cuda(), torch.from_numpy(B16).cuda(), N)
+    torch.cuda.synchronize()
+    ev, el = O.topk_rows(C.cpu().numpy(), 20)
+    eng = R.TopKEngine(B8, k=20, mode="fp16")
+    for S in (1, 3):
+        v, l = fused(torch, eng, Q, splits=S)
+        np.testing.assert_array_equal(l, el)
+        np.testing.assert_array_equal(v, ev)
+
+
+def test_fused_rescore_bitexact(torch_gpu):
+    torch = torch_gpu
+    M, N, K, k, kp = 200, 3000, 2048, 20, 32
+    Q8, Q32 = unit_rows_e4m3(M, K, 31)
+    B, _ = unit_rows_e4m3(N, K, 32)
+    C = gemm_c(torch, Q8, B)
+    _, cand = O.topk_rows(C, kp)
+    eng = R.TopKEngine(B, k=k, kprime=kp, mode="fp8_rescore")
+    v, l = fused(torch, eng, Q8, Q32)
+    for r in range(M):
+        ev, el = O.rescore_rank(Q32[r], B, cand[r], k)
+        np.testing.assert_array_equal(l[r], el)
+        np.testing.assert_array_equal(v[r], ev)
+    # rescored values vs the FP64 dot of the fp32 query with the stored columns
+    deq = O.e4m3_table()[B].astype(np.float64)
+    ref = Q32.astype(np.float64) @ deq.T
+    got = ref[np.arange(M)[:, None], l - 1]
+    assert np.max(np.abs(got - v)) < 1e-5
+
+
+def test_rowtopk_merge_dropin(torch_gpu):
+    torch = torch_gpu
+    M, N, K, w, k = 257, 1000, 512, 300, 20
+    Q = rand_e4m3((M, K), 0.05, 41)
+    B = rand_e4m3((N, K), 0.05, 42)
+    dq, db = torch.from_numpy(Q).cuda(), torch.from_numpy(B).cuda()
+    for cdt in (torch.float32, torch.float16):
+        Dv = torch.full((k, M), float("-inf"), device="cuda").t()
+        Dl = torch.zeros(k, M, dtype=torch.int32, device="cuda").t()
+        Call = np.zeros((M, N), dtype=np.float32)
+        for lo in range(0, N, w):
+            wi = min(N, lo + w) - lo
+            buf = torch.zeros(w, M, dtype=cdt, device="cuda").t()  # engine-style parent
+            R.fp8_gemm_hip(buf[:, :wi], dq, db, wi, b_col0=lo)
+            R.launch_seg_rowtopk_merge(Dv, Dl, buf[:, :wi], lo, k)
+            torch.cuda.synchronize()
+            Call[:, lo:lo + wi] = buf[:, :wi].float().cpu().numpy()
+        ev, el = O.topk_rows(Call, k)
+        np.testing.assert_array_equal(Dl.cpu().numpy(), el)
+        np.testing.assert_array_equal(Dv.cpu().numpy(), ev)
