@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--fp16-steps", type=int, default=1)
     ap.add_argument("--cpu-reads", type=int, default=0, help="0 = size the CPU sample to ~15 s")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 1 GiB device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -79,6 +81,13 @@ def main():
     import rotormap_amd as R
     from rotormap_amd import _lib as L
 
+    if args.calibrate_copy:
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     t0 = time.time()
     log = lambda *a: print(f"[bench {time.time() - t0:7.1f}s]", *a, file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")

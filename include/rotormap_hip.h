@@ -170,15 +170,18 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words,
                    int mode, float *vals, int32_t *locs, int64_t ldo, void *stream);
 
 /* ---- tile plan / autotuner ------------------------------------------------
- * Column-split count used for a batch of M rows (0 = the built-in rule).
- * rm_autotune times candidate split counts for each M bucket on `idx` and
- * keeps the fastest in the context (results also returned, host arrays of
- * nbuckets entries: bucket row counts in, chosen splits and ms out). */
+ * A plan is (column splits S, query tiles grouped per XCD qg) per batch-size
+ * bucket ceil(log2 M).  rm_plan_splits returns the split count a batch of M
+ * rows will use.  rm_autotune times the candidate plans of each bucket on
+ * `idx` (tuning queries: the index's own columns) and keeps the fastest in
+ * the context (host arrays of nbuckets entries: bucket row counts in; chosen
+ * S, qg and ms out). */
 int rm_plan_splits(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode);
-/* Pin the split count for batches in M's bucket (ceil(log2 M)); 0 clears. */
-int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int splits);
+/* Pin the plan for batches in M's bucket (ceil(log2 M)): the column split
+ * count (0 clears the entry) and the query tiles grouped per XCD (0 = all). */
+int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int splits, int qgroup);
 int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, int nbuckets,
-                int mode, int32_t *best_splits, float *best_ms);
+                int mode, int32_t *best_splits, int32_t *best_qgroup, float *best_ms);
 
 /* ---- kernel timing ---------------------------------------------------------
  * When enabled, the context records a pair of HIP events around every launch

@@ -73,11 +73,12 @@ int table_get(rm_ctx *ctx, int k, int s, int m, const int32_t **out) {
 }
 
 // Column splits for M query rows over N index columns: enough (row tile,
-// split) items for two resident workgroups per CU, never more splits than
-// column tiles; the autotuned plan overrides per M bucket.
-int default_splits(rm_ctx *ctx, int64_t M, int64_t N) {
-    int64_t R = cdiv(M, kBR), tiles = cdiv(N, kBI);
-    int64_t want = cdiv(2 * (int64_t)ctx->num_cus, R);
+// split) items for one resident workgroup per CU (the fused kernel holds a
+// CU's LDS), never more splits than column tiles; the autotuned plan
+// overrides per M bucket.
+int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp) {
+    int64_t R = cdiv(M, rows_per_tile(kp)), tiles = cdiv(N, kBI);
+    int64_t want = cdiv((int64_t)ctx->num_cus, R);
     int64_t S = std::max<int64_t>(1, std::min<int64_t>(want, tiles));
     return (int)std::min<int64_t>(S, 256);
 }
@@ -88,16 +89,24 @@ static int bucket_of(int64_t M) {
     return b;  // ceil(log2 M)
 }
 
-static int splits_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode) {
+// Tile plan for a batch: (splits, query tiles per XCD group).  The built-in
+// rule fills the CUs with one workgroup each and groups 4 query tiles per
+// XCD; the autotuned plan (rm_autotune / rm_plan_set) overrides per bucket.
+static std::pair<int, int> plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
     auto it = ctx->plan.find({mode == RM_MODE_FP16 ? 1 : 0, bucket_of(M)});
     int64_t tiles = cdiv(idx->N, kBI);
-    if (it != ctx->plan.end()) return (int)std::max<int64_t>(1, std::min<int64_t>(it->second, tiles));
-    return default_splits(ctx, M, idx->N);
+    if (it != ctx->plan.end())
+        return {(int)std::max<int64_t>(1, std::min<int64_t>(it->second.first, tiles)), it->second.second};
+    return {default_splits(ctx, M, idx->N, kp), 0};
+}
+
+static int splits_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
+    return plan_for(ctx, idx, M, mode, kp).first;
 }
 
 int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
               const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
-              float *vals, int32_t *locs, int64_t ldo, int S, hipStream_t st) {
+              float *vals, int32_t *locs, int64_t ldo, int S, int qg, hipStream_t st) {
     const int KP = (mode == RM_MODE_FP8_RESCORE) ? kprime : ktop;
     const int64_t K = idx->rdim;
     const int esz = elem == 0 ? 1 : 2;
@@ -121,7 +130,9 @@ int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes
     a.N = idx->N;
     a.Kb = (int)(K * esz);
     a.S = S;
-    a.R = cdiv(M, kBR);
+    a.R = cdiv(M, rows_per_tile(KP));
+    a.qg = (qg <= 0 || qg > a.R) ? (int)a.R : qg;
+    a.ngroups = (int)cdiv(a.R, a.qg);
     a.tiles = tiles;
     a.col_base = 0;
     a.part_v = (float *)pv;
@@ -302,7 +313,9 @@ int rm_gemm_chunk(rm_ctx *ctx, void *C, int c_dtype, int64_t ldc, const void *A,
     a.M = M;
     a.N = wi;
     a.Kb = (int)(K * esz);
-    a.R = cdiv(M, kBR);
+    a.R = cdiv(M, 256);
+    a.qg = (int)a.R;
+    a.ngroups = 1;
     a.tiles = cdiv(wi, kBI);
     a.S = (int)a.tiles;
     a.C = C;
@@ -408,8 +421,9 @@ int rm_topk_batch(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq, 
     if (mode == RM_MODE_FP8_RESCORE) RM_CHECK(Q32 && ldq32 >= idx->rdim, RM_ERR_ARG, "Q32 required");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     RM_HIP(hipSetDevice(ctx->device));
+    const auto pl = plan_for(ctx, idx, M, mode, mode == RM_MODE_FP8_RESCORE ? kprime : ktop);
     return topk_core(ctx, idx, Q, ldq_b, elem, Q32, ldq32, M, ktop, kprime, mode, vals, locs, ldo,
-                     splits_for(ctx, idx, M, mode), pick_stream(ctx, stream));
+                     pl.first, pl.second, pick_stream(ctx, stream));
 }
 
 // encode n fragments straight into the query layouts the GEMM reads
@@ -452,8 +466,9 @@ int rm_map_batch(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_
     float *q32;
     if ((rc = encode_queries(ctx, words, n, k, s, m, c, mode, rdim, norms, &q, &q32, st))) return rc;
     const int elem = mode == RM_MODE_FP16 ? 1 : 0;
+    const auto pl = plan_for(ctx, idx, n, mode, mode == RM_MODE_FP8_RESCORE ? kprime : ktop);
     return topk_core(ctx, idx, q, rdim * (elem ? 2 : 1), elem, q32, rdim, n, ktop, kprime, mode,
-                     vals, locs, ldo, splits_for(ctx, idx, n, mode), st);
+                     vals, locs, ldo, pl.first, pl.second, st);
 }
 
 int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, const int64_t *offs,
@@ -475,20 +490,21 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, cons
 int rm_plan_splits(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode) {
     if (!ctx || !idx || M <= 0) return 0;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    return splits_for(ctx, idx, M, mode);
+    return splits_for(ctx, idx, M, mode, mode == RM_MODE_FP8_RESCORE ? 32 : 20);
 }
 
-int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int splits) {
-    RM_CHECK(ctx && M >= 1 && splits >= 0 && splits <= 256, RM_ERR_ARG, "bad plan entry");
+int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int splits, int qgroup) {
+    RM_CHECK(ctx && M >= 1 && splits >= 0 && splits <= 256 && qgroup >= 0, RM_ERR_ARG,
+             "bad plan entry");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     auto key = std::make_pair(mode == RM_MODE_FP16 ? 1 : 0, bucket_of(M));
     if (splits == 0) ctx->plan.erase(key);
-    else ctx->plan[key] = splits;
+    else ctx->plan[key] = {splits, qgroup};
     return RM_OK;
 }
 
 int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, int nbuckets,
-                int mode, int32_t *best_splits, float *best_ms) {
+                int mode, int32_t *best_splits, int32_t *best_qgroup, float *best_ms) {
     RM_CHECK(ctx && idx && bucket_rows && nbuckets >= 0, RM_ERR_ARG, "NULL argument");
     int rc;
     if ((rc = check_query(idx, 20, 32, mode))) return rc;
@@ -497,7 +513,9 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
     hipStream_t st = ctx->stream;
     const int elem = mode == RM_MODE_FP16 ? 1 : 0;
     const int64_t rdim = idx->rdim;
+    const int esz = elem ? 2 : 1;
     const int64_t tiles = cdiv(idx->N, kBI);
+    const int kp = mode == RM_MODE_FP8_RESCORE ? 32 : 20;
     hipEvent_t e0, e1;
     RM_HIP(hipEventCreate(&e0));
     RM_HIP(hipEventCreate(&e1));
@@ -505,40 +523,51 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
         const int64_t M = bucket_rows[b];
         RM_CHECK(M >= 1, RM_ERR_ARG, "bucket rows must be >= 1");
         void *q, *q32 = nullptr, *vo, *lo;
-        if ((rc = ws_get(ctx, "tune_q", (size_t)M * rdim * (elem ? 2 : 1), &q))) return rc;
+        if ((rc = ws_get(ctx, "tune_q", (size_t)M * rdim * esz, &q))) return rc;
         if (mode == RM_MODE_FP8_RESCORE && (rc = ws_get(ctx, "tune_q32", (size_t)M * rdim * 4, &q32)))
             return rc;
         if ((rc = ws_get(ctx, "tune_v", (size_t)M * 20 * 4, &vo))) return rc;
         if ((rc = ws_get(ctx, "tune_l", (size_t)M * 20 * 4, &lo))) return rc;
-        RM_HIP(hipMemsetAsync(q, 0x1c, (size_t)M * rdim * (elem ? 2 : 1), st));
-        if (q32) RM_HIP(hipMemsetAsync(q32, 0, (size_t)M * rdim * 4, st));
-        const int64_t R = cdiv(M, kBR);
-        std::vector<int> cands;
+        // tuning queries: index columns themselves (unit-energy embeddings
+        // with one perfect match each, like reads), repeated when M > N
+        const uint8_t *src = elem ? (const uint8_t *)idx->b16 : idx->b8;
+        for (int64_t r = 0; r < M; r += idx->N) {
+            const int64_t n = std::min<int64_t>(idx->N, M - r);
+            RM_HIP(hipMemcpyAsync((uint8_t *)q + r * rdim * esz, src, (size_t)(n * rdim * esz),
+                                  hipMemcpyDeviceToDevice, st));
+        }
+        if (q32 && (rc = launch_dequantize((float *)q32, (const uint8_t *)q, M * rdim, st))) return rc;
+        const int64_t R = cdiv(M, rows_per_tile(kp));
+        std::vector<std::pair<int, int>> cands;
         for (int S = 1; S <= 256 && S <= tiles; S *= 2) {
             if (R * S > 64LL * ctx->num_cus && S > 1) break;
-            cands.push_back(S);
+            for (int qg : {0, 8, 4, 2}) {
+                if (qg && qg >= R) continue;
+                cands.push_back({S, qg});
+            }
         }
-        int bestS = cands.empty() ? 1 : cands[0];
+        std::pair<int, int> best = cands.empty() ? std::make_pair(1, 0) : cands[0];
         float bestT = 1e30f;
-        for (int S : cands) {
-            if ((rc = topk_core(ctx, idx, q, rdim * (elem ? 2 : 1), elem, (float *)q32, rdim, M, 20,
-                                32, mode, (float *)vo, (int32_t *)lo, M, S, st)))
-                return rc;  // warm-up
+        for (auto c : cands) {
+            auto run = [&]() {
+                return topk_core(ctx, idx, q, rdim * esz, elem, (float *)q32, rdim, M, 20, kp, mode,
+                                 (float *)vo, (int32_t *)lo, M, c.first, c.second, st);
+            };
+            if ((rc = run())) return rc;  // warm-up
             RM_HIP(hipEventRecord(e0, st));
             const int reps = 2;
             for (int r = 0; r < reps; ++r)
-                if ((rc = topk_core(ctx, idx, q, rdim * (elem ? 2 : 1), elem, (float *)q32, rdim, M,
-                                    20, 32, mode, (float *)vo, (int32_t *)lo, M, S, st)))
-                    return rc;
+                if ((rc = run())) return rc;
             RM_HIP(hipEventRecord(e1, st));
             RM_HIP(hipEventSynchronize(e1));
             float ms = 0;
             RM_HIP(hipEventElapsedTime(&ms, e0, e1));
             ms /= reps;
-            if (ms < bestT) { bestT = ms; bestS = S; }
+            if (ms < bestT) { bestT = ms; best = c; }
         }
-        ctx->plan[{elem, bucket_of(M)}] = bestS;
-        if (best_splits) best_splits[b] = bestS;
+        ctx->plan[{elem, bucket_of(M)}] = best;
+        if (best_splits) best_splits[b] = best.first;
+        if (best_qgroup) best_qgroup[b] = best.second;
         if (best_ms) best_ms[b] = bestT;
     }
     hipEventDestroy(e0);

@@ -15,8 +15,8 @@
 // ONE query (lane % 32) and 16 index columns -- the per-query top-k needs no
 // cross-lane traffic to filter.
 //
-// Tile: 256 index rows x 128 queries per workgroup (4 waves in 2x2, wave
-// tile 128 x 64 = 4 x 2 blocks of v_mfma_f32_32x32x64_f8f6f4, or 2 x
+// Tile: 256 index rows x 256 queries per workgroup (4 waves in 2x2, wave
+// tile 128 x 128 = 4 x 4 blocks of v_mfma_f32_32x32x64_f8f6f4, or 2 x
 // v_mfma_f32_32x32x16_f16 per 64-byte stage for the FP16 path), K streamed
 // through double-buffered LDS in 64-byte stages (80-byte row pitch: the
 // ds_read_b128 fragment reads of 8 consecutive rows hit 8 distinct 4-bank
@@ -40,48 +40,65 @@ typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
-struct StageRegs {
-    uint4 v[6];
+// Tile: 256 index rows x BR queries, BR = 64*BB (BB = 4: 256, BB = 2: 128).
+// Four waves in 2 x 2; a wave owns 128 index rows x 32*BB queries as 4 x BB
+// blocks of 32 x 32.
+template <int WR, int BB>
+struct Tile {
+    static constexpr int NT = 128 * WR;          // threads: 2 x WR waves
+    static constexpr int BR = 32 * WR * BB;      // queries per tile
+    static constexpr int ROWS = kBI + BR;        // LDS rows per stage buffer
+    static constexpr int CHUNKS = ROWS * 4 / NT; // 16-byte chunks per thread per stage
+    static constexpr int STAGE_BYTES = ROWS * kRow;
 };
 
-// global -> registers for one 64-byte K stage: index rows 0..255 (4 x 16 B
-// chunks each, 1024 chunks) and query rows 0..127 (512 chunks); out-of-range
-// rows read as zeros.
-__device__ __forceinline__ void g_load(StageRegs &st, const GemmArgs &a, int64_t i0, int64_t r0,
-                                       int kb, int tid) {
+template <int WR, int BB>
+struct StageRegs {
+    uint4 v[Tile<WR, BB>::CHUNKS];
+};
+
+// Buffer descriptors of a tile's operand rows: base at the tile's first row,
+// num_records = the in-range bytes, so loads past the last row return zeros
+// (no per-load branch; 32-bit offsets within the tile).
+struct TileSrc {
+    __amdgpu_buffer_rsrc_t db, q;
+    int ldb, ldq;
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const uint8_t *base, int64_t row0,
+                                                            int64_t nrows, int64_t rows_max,
+                                                            int64_t ld) {
+    int64_t n = nrows - row0;
+    n = n < 0 ? 0 : (n > rows_max ? rows_max : n);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(base + row0 * ld), (short)0, (int)(n * ld),
+                                             0x00020000);
+}
+
+// global -> registers for one 64-byte K stage: chunk j of thread tid covers
+// row tid/4 + (NT/4)*j (index rows first, then query rows), bytes
+// (tid%4)*16 .. +16 of the stage.
+template <int WR, int BB>
+__device__ __forceinline__ void g_load(StageRegs<WR, BB> &st, const TileSrc &ts, int tid, int kb) {
+    constexpr int RS = Tile<WR, BB>::NT / 4;  // rows per chunk step
+    const int r = tid >> 2, seg = (tid & 3) * 16;
+    const int vdb = r * ts.ldb + seg, vq = r * ts.ldq + seg;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        int c = tid + 256 * j;
-        int row = c >> 2, seg = c & 3;
-        int64_t gi = i0 + row;
-        uint4 z = make_uint4(0, 0, 0, 0);
-        if (gi < a.N) z = *(const uint4 *)(a.db + gi * a.ldb + kb + seg * 16);
-        st.v[j] = z;
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        int c = tid + 256 * j;
-        int row = c >> 2, seg = c & 3;
-        int64_t gr = r0 + row;
-        uint4 z = make_uint4(0, 0, 0, 0);
-        if (gr < a.M) z = *(const uint4 *)(a.q + gr * a.ldq + kb + seg * 16);
-        st.v[4 + j] = z;
+    for (int j = 0; j < Tile<WR, BB>::CHUNKS; ++j) {
+        if (j < kBI / RS)
+            st.v[j] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                    ts.db, vdb, j * RS * ts.ldb + kb, 0));
+        else
+            st.v[j] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                    ts.q, vq, (j - kBI / RS) * RS * ts.ldq + kb, 0));
     }
 }
 
-__device__ __forceinline__ void s_store(const StageRegs &st, uint8_t *sb, int tid) {
+template <int WR, int BB>
+__device__ __forceinline__ void s_store(const StageRegs<WR, BB> &st, uint8_t *sb, int tid) {
+    constexpr int RS = Tile<WR, BB>::NT / 4;
+    uint8_t *p = sb + (tid >> 2) * kRow + (tid & 3) * 16;  // chunk j adds RS rows
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        int c = tid + 256 * j;
-        int row = c >> 2, seg = c & 3;
-        *(uint4 *)(sb + row * kRow + seg * 16) = st.v[j];
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        int c = tid + 256 * j;
-        int row = c >> 2, seg = c & 3;
-        *(uint4 *)(sb + (kBI + row) * kRow + seg * 16) = st.v[4 + j];
-    }
+    for (int j = 0; j < Tile<WR, BB>::CHUNKS; ++j) *(uint4 *)(p + j * RS * kRow) = st.v[j];
 }
 
 __device__ __forceinline__ v8i ld_v8i(const uint8_t *p) {
@@ -91,114 +108,159 @@ __device__ __forceinline__ v8i ld_v8i(const uint8_t *p) {
     r[4] = (int)y.x; r[5] = (int)y.y; r[6] = (int)y.z; r[7] = (int)y.w;
     return r;
 }
-__device__ __forceinline__ v8h ld_v8h(const uint8_t *p) {
-    uint4 x = *(const uint4 *)p;
-    return __builtin_bit_cast(v8h, x);
+
+// One stage's operand fragments of a wave: lane l feeds row (l % 32) of each
+// block with K bytes [(l/32)*32, +32) of the stage (fp8), or halves
+// [kk*16 + (l/32)*8, +8) per 16-wide f16 step kk: the same K slice for both
+// operands, so every product pairs matching K indices.
+template <int BB>
+struct Frags {
+    uint4 a[4][2];
+    uint4 b[BB][2];
+};
+
+template <int ELEM>
+__device__ __forceinline__ void ld_pair(uint4 (&d)[2], const uint8_t *row, int lane) {
+    const int hl = lane >> 5;
+    const int o0 = ELEM == 0 ? hl * 32 : hl * 16;            // first 16 bytes
+    const int o1 = ELEM == 0 ? hl * 32 + 16 : 32 + hl * 16;  // second 16 bytes
+    d[0] = *(const uint4 *)(row + o0);
+    d[1] = *(const uint4 *)(row + o1);
 }
 
-// One 64-byte K stage of MFMAs for this wave's 4 x 2 blocks.  Lane l feeds
-// row (l % 32) of its block with K bytes [(l/32)*32, +32) of the stage (fp8)
-// or, per 16-wide f16 step kk, halves [kk*16 + (l/32)*8, +8): the same K
-// slice for both operands, so every product pairs matching K indices.
-template <int ELEM>
-__device__ __forceinline__ void mma_stage(v16f (&acc)[4][2], const uint8_t *sb, int wi, int wr,
-                                          int lane) {
-    const int rl = lane & 31, hl = lane >> 5;
-    if constexpr (ELEM == 0) {
-        v8i fa[4], fb[2];
+template <int ELEM, int BB>
+__device__ __forceinline__ void ld_frags_b(Frags<BB> &f, const uint8_t *sb, int wr, int lane) {
+    // wave wr owns queries [wr*32*BB, (wr+1)*32*BB)
 #pragma unroll
-        for (int ai = 0; ai < 4; ++ai)
-            fa[ai] = ld_v8i(sb + (wi * 128 + ai * 32 + rl) * kRow + hl * 32);
+    for (int bj = 0; bj < BB; ++bj)
+        ld_pair<ELEM>(f.b[bj], sb + (kBI + wr * 32 * BB + bj * 32 + (lane & 31)) * kRow, lane);
+}
+
+template <int ELEM, int BB, int A0, int A1>
+__device__ __forceinline__ void ld_frags_a(Frags<BB> &f, const uint8_t *sb, int wi, int lane) {
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-            fb[bj] = ld_v8i(sb + (kBI + wr * 64 + bj * 32 + rl) * kRow + hl * 32);
+    for (int ai = A0; ai < A1; ++ai)
+        ld_pair<ELEM>(f.a[ai], sb + (wi * 128 + ai * 32 + (lane & 31)) * kRow, lane);
+}
+
+struct Bytes32 {
+    uint4 lo, hi;
+};
+
+template <int ELEM, int BB, int A0, int A1>
+__device__ __forceinline__ void mma_part(v16f (&acc)[4][BB], const Frags<BB> &f) {
 #pragma unroll
-        for (int ai = 0; ai < 4; ++ai)
+    for (int ai = A0; ai < A1; ++ai)
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
+        for (int bj = 0; bj < BB; ++bj) {
+            if constexpr (ELEM == 0) {
+                const v8i fa = __builtin_bit_cast(v8i, (Bytes32{f.a[ai][0], f.a[ai][1]}));
+                const v8i fb = __builtin_bit_cast(v8i, (Bytes32{f.b[bj][0], f.b[bj][1]}));
                 acc[ai][bj] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                    fa[ai], fb[bj], acc[ai][bj], 0, 0, 0, 0, 0, 0);
-    } else {
+                    fa, fb, acc[ai][bj], 0, 0, 0, 0, 0, 0);
+            } else {
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-            v8h fa[4], fb[2];
-#pragma unroll
-            for (int ai = 0; ai < 4; ++ai)
-                fa[ai] = ld_v8h(sb + (wi * 128 + ai * 32 + rl) * kRow + kk * 32 + hl * 16);
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-                fb[bj] = ld_v8h(sb + (kBI + wr * 64 + bj * 32 + rl) * kRow + kk * 32 + hl * 16);
-#pragma unroll
-            for (int ai = 0; ai < 4; ++ai)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj)
-                    acc[ai][bj] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[ai], fb[bj],
-                                                                         acc[ai][bj], 0, 0, 0);
+                for (int kk = 0; kk < 2; ++kk)
+                    acc[ai][bj] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                        __builtin_bit_cast(v8h, f.a[ai][kk]), __builtin_bit_cast(v8h, f.b[bj][kk]),
+                        acc[ai][bj], 0, 0, 0);
+            }
         }
-    }
 }
 
-// Full-K product of one (256 index x 128 query) tile into acc.
-template <int ELEM>
-__device__ __forceinline__ void tile_product(v16f (&acc)[4][2], const GemmArgs &a, int64_t i0,
+// Full-K product of one (256 index x BR query) tile into acc, one barrier per
+// 64-byte stage.  After the barrier a wave reads the query fragments and the
+// first two index fragments of stage s, stores stage s+1 (already in
+// registers) into the other buffer and issues stage s+2's global loads; the
+// last two index fragments are read while the first half's MFMAs run.
+template <int ELEM, int WR, int BB>
+__device__ __forceinline__ void tile_product(v16f (&acc)[4][BB], const GemmArgs &a, int64_t i0,
                                              int64_t r0, uint8_t *sb0, uint8_t *sb1, int tid,
                                              int wi, int wr, int lane) {
+    TileSrc ts;
+    ts.db = rows_rsrc(a.db, i0, a.N, kBI, a.ldb);
+    ts.q = rows_rsrc(a.q, r0, a.M, Tile<WR, BB>::BR, a.ldq);
+    ts.ldb = (int)a.ldb;
+    ts.ldq = (int)a.ldq;
 #pragma unroll
     for (int ai = 0; ai < 4; ++ai)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
+        for (int bj = 0; bj < BB; ++bj)
 #pragma unroll
             for (int i = 0; i < 16; ++i) acc[ai][bj][i] = 0.0f;
     const int nk = a.Kb / kBK;
-    StageRegs st;
-    g_load(st, a, i0, r0, 0, tid);
-    s_store(st, sb0, tid);
-    __syncthreads();
-    for (int ks = 0; ks < nk; ++ks) {
-        uint8_t *cur = (ks & 1) ? sb1 : sb0;
-        uint8_t *nxt = (ks & 1) ? sb0 : sb1;
-        const bool more = ks + 1 < nk;
-        if (more) g_load(st, a, i0, r0, (ks + 1) * kBK, tid);
-        mma_stage<ELEM>(acc, cur, wi, wr, lane);
-        if (more) s_store(st, nxt, tid);
+    StageRegs<WR, BB> g;
+    Frags<BB> f;
+    g_load<WR, BB>(g, ts, tid, 0);
+    s_store<WR, BB>(g, sb0, tid);
+    if (nk > 1) g_load<WR, BB>(g, ts, tid, kBK);
+    for (int s = 0; s < nk; ++s) {
+        uint8_t *cur = (s & 1) ? sb1 : sb0;
+        uint8_t *nxt = (s & 1) ? sb0 : sb1;
         __syncthreads();
+        ld_frags_b<ELEM, BB>(f, cur, wr, lane);
+        ld_frags_a<ELEM, BB, 0, 2>(f, cur, wi, lane);
+        if (s + 1 < nk) s_store<WR, BB>(g, nxt, tid);
+        if (s + 2 < nk) g_load<WR, BB>(g, ts, tid, (s + 2) * kBK);
+        mma_part<ELEM, BB, 0, 2>(acc, f);
+        ld_frags_a<ELEM, BB, 2, 4>(f, cur, wi, lane);
+        mma_part<ELEM, BB, 2, 4>(acc, f);
     }
 }
 
+// Block-wide OR of a predicate with one barrier: waves with a true lane set
+// flag slot `k % 3`; thread 0 clears the next slot for a later vote.
+__device__ __forceinline__ bool block_any(bool p, int *flags, int &k) {
+    const int slot = k % 3;
+    if (threadIdx.x == 0) flags[(slot + 1) % 3] = 0;
+    if (__builtin_amdgcn_read_exec() && __ballot(p) && (threadIdx.x & 63) == __builtin_ffsll(__ballot(p)) - 1)
+        atomicOr(&flags[slot], 1);
+    __syncthreads();
+    ++k;
+    return flags[slot] != 0;
+}
+
 // Accumulator i of block (ai, bj) at lane l: index row
-// wi*128 + ai*32 + 8*(i/4) + 4*(l/32) + (i%4), query wr*64 + bj*32 + l%32
+// wi*128 + ai*32 + 8*(i/4) + 4*(l/32) + (i%4), query wr*32*BB + bj*32 + l%32
 // (the 32x32 MFMA output layout).
 __device__ __forceinline__ int acc_row(int wi, int ai, int i, int lane) {
     return wi * 128 + ai * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
 }
 
-// Decode a workgroup's (row tile, split) from the launch: XCD-grouped grid.
+// Decode a workgroup's (row tile, split) from the launch.  Dispatch is
+// round-robin over the 8 XCDs, so workgroup g runs on XCD g % 8; the logical
+// item L = (g % 8) * (G / 8) + g / 8 makes an XCD's concurrently resident
+// workgroups consecutive in L.  Items are ordered (query group, split,
+// query tile in group): the resident workgroups of an XCD then cover qg query
+// tiles x S_xcd splits, so each query tile and each index tile is fetched
+// into that XCD's L2 once and shared.
 __device__ __forceinline__ bool decode_item(const GemmArgs &a, int64_t &rt, int &split) {
     const int64_t G = gridDim.x, g = blockIdx.x;
     const int64_t L = (g & 7) * (G >> 3) + (g >> 3);
-    if (L >= a.R * a.S) return false;
-    split = (int)(L / a.R);
-    rt = L % a.R;
-    return true;
+    const int64_t per = (int64_t)a.qg * a.S;
+    const int64_t grp = L / per, rem = L % per;
+    split = (int)(rem / a.qg);
+    rt = grp * a.qg + rem % a.qg;
+    return grp * per < (int64_t)a.ngroups * per && rt < a.R;
 }
 
-template <int ELEM>
-__global__ __launch_bounds__(256, 1) void gemm_store_kernel(GemmArgs a) {
+template <int ELEM, int WR, int BB>
+__global__ __launch_bounds__(128 * WR, 1) void gemm_store_kernel(GemmArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     int64_t rt;
     int split;
     if (!decode_item(a, rt, split)) return;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid & 1, wr = wid >> 1;
-    uint8_t *sb0 = smem, *sb1 = smem + (kBI + kBR) * kRow;
-    const int64_t r0 = rt * kBR;
+    uint8_t *sb0 = smem, *sb1 = smem + Tile<WR, BB>::STAGE_BYTES;
+    const int64_t r0 = rt * Tile<WR, BB>::BR;
     const int64_t t0 = (int64_t)split * a.tiles / a.S, t1 = (int64_t)(split + 1) * a.tiles / a.S;
-    v16f acc[4][2];
+    v16f acc[4][BB];
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
-        tile_product<ELEM>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
+        tile_product<ELEM, WR, BB>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            const int64_t q = r0 + wr * 64 + bj * 32 + (lane & 31);
+        for (int bj = 0; bj < BB; ++bj) {
+            const int64_t q = r0 + wr * 32 * BB + bj * 32 + (lane & 31);
             if (q >= a.M) continue;
 #pragma unroll
             for (int ai = 0; ai < 4; ++ai)
@@ -212,86 +274,117 @@ __global__ __launch_bounds__(256, 1) void gemm_store_kernel(GemmArgs a) {
                     }
                 }
         }
+        __syncthreads();  // the next tile's prologue refills the stage buffers
     }
 }
 
-template <int ELEM>
-__global__ __launch_bounds__(256, 1) void gemm_topk_kernel(GemmArgs a) {
+template <int ELEM, int WR, int BB>
+__global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
+    constexpr int BR = Tile<WR, BB>::BR;
+    constexpr int NT = Tile<WR, BB>::NT;
     extern __shared__ __align__(16) uint8_t smem[];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid & 1, wr = wid >> 1;
     const int KP = a.KP, LP = KP + 1, CP = kCap + 1;
-    uint8_t *sb0 = smem, *sb1 = smem + (kBI + kBR) * kRow;
-    float *lv = (float *)(smem + 2 * (kBI + kBR) * kRow);  // kBR x (KP+1)
-    int32_t *lc = (int32_t *)(lv + kBR * LP);
-    float *cv = (float *)(lc + kBR * LP);                  // kBR x (kCap+1)
-    int32_t *cc = (int32_t *)(cv + kBR * CP);
-    float *tauv = (float *)(cc + kBR * CP);                // kBR
-    int *cnt = (int *)(tauv + kBR);                        // kBR
+    uint8_t *sb0 = smem, *sb1 = smem + Tile<WR, BB>::STAGE_BYTES;
+    // candidate slots alias the stage buffers: they are used only between a
+    // tile's last MFMA and the next tile's first stage store
+    float *cv = (float *)smem;                              // BR x (kCap+1)
+    int32_t *cc = (int32_t *)(cv + BR * CP);
+    float *lv = (float *)(smem + 2 * Tile<WR, BB>::STAGE_BYTES); // BR x (KP+1)
+    int32_t *lc = (int32_t *)(lv + BR * LP);
+    float *tauv = (float *)(lc + BR * LP);                  // BR
+    int *cnt = (int *)(tauv + BR);                          // BR
+    int *flags = cnt + BR;                                  // 3 vote slots
 
     int64_t rt;
     int split;
     if (!decode_item(a, rt, split)) return;
-    const int64_t r0 = rt * kBR;
+    const int64_t r0 = rt * BR;
     const int64_t t0 = (int64_t)split * a.tiles / a.S, t1 = (int64_t)(split + 1) * a.tiles / a.S;
 
-    for (int i = tid; i < kBR * LP; i += 256) { lv[i] = -INFINITY; lc[i] = INT_MAX; }
-    if (tid < kBR) { tauv[tid] = -INFINITY; cnt[tid] = 0; }
+    for (int i = tid; i < BR * LP; i += NT) { lv[i] = -INFINITY; lc[i] = INT_MAX; }
+    for (int i = tid; i < BR; i += NT) { tauv[i] = -INFINITY; cnt[i] = 0; }
+    if (tid < 3) flags[tid] = 0;
+    int vote = 0;
     // (the first tile's barriers order this before any use)
 
-    const int ql0 = wr * 64 + (lane & 31), ql1 = ql0 + 32;  // this lane's two queries
-    const bool qv0 = r0 + ql0 < a.M, qv1 = r0 + ql1 < a.M;
-    v16f acc[4][2];
+    int ql[BB];
+    bool qv[BB];
+#pragma unroll
+    for (int bj = 0; bj < BB; ++bj) {
+        ql[bj] = wr * 32 * BB + bj * 32 + (lane & 31);  // this lane's queries
+        qv[bj] = r0 + ql[bj] < a.M;
+    }
+    v16f acc[4][BB];
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
-        tile_product<ELEM>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
+        tile_product<ELEM, WR, BB>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
+        const bool full = i0 + kBI <= a.N;
 
-        // --- filter: flag values strictly above tau (a later column can only
-        // beat an equal list entry by value, never by the column tie-break)
-        float tau0 = tauv[ql0], tau1 = tauv[ql1];
-        uint64_t pend0 = 0, pend1 = 0;
+        // --- filter: a query's values are compared one by one only when its
+        // tile maximum beats tau.  Strictly above tau: a later column can
+        // only beat an equal list entry by value, never by the column rule.
+        uint64_t pend[BB];
 #pragma unroll
-        for (int ai = 0; ai < 4; ++ai)
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const bool ok = i0 + acc_row(wi, ai, i, lane) < a.N;
-                const uint64_t bit = 1ull << (ai * 16 + i);
-                if (ok && qv0 && acc[ai][0][i] > tau0) pend0 |= bit;
-                if (ok && qv1 && acc[ai][1][i] > tau1) pend1 |= bit;
-            }
-        while (__syncthreads_or((pend0 | pend1) != 0ull)) {
-            // --- append flagged values to the per-query candidate slots
+        for (int bj = 0; bj < BB; ++bj) {
+            pend[bj] = 0;
+            const float tau = tauv[ql[bj]];
+            float mx = -INFINITY;
 #pragma unroll
             for (int ai = 0; ai < 4; ++ai)
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    const uint64_t bit = 1ull << (ai * 16 + i);
-                    const int32_t col = (int32_t)(a.col_base + i0 + acc_row(wi, ai, i, lane));
-                    if (pend0 & bit) {
-                        int slot = atomicAdd(&cnt[ql0], 1);
-                        if (slot < kCap) {
-                            cv[ql0 * CP + slot] = acc[ai][0][i];
-                            cc[ql0 * CP + slot] = col;
-                            pend0 &= ~bit;
-                        }
+                    const bool ok = full || (i0 + acc_row(wi, ai, i, lane) < a.N);
+                    mx = fmaxf(mx, ok ? acc[ai][bj][i] : -INFINITY);
+                }
+            if (qv[bj] && mx > tau) {
+#pragma unroll
+                for (int ai = 0; ai < 4; ++ai)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const bool ok = full || (i0 + acc_row(wi, ai, i, lane) < a.N);
+                        if (ok && acc[ai][bj][i] > tau) pend[bj] |= 1ull << (ai * 16 + i);
                     }
-                    if (pend1 & bit) {
-                        int slot = atomicAdd(&cnt[ql1], 1);
-                        if (slot < kCap) {
-                            cv[ql1 * CP + slot] = acc[ai][1][i];
-                            cc[ql1 * CP + slot] = col;
-                            pend1 &= ~bit;
+            }
+        }
+        uint64_t any = 0;
+#pragma unroll
+        for (int bj = 0; bj < BB; ++bj) any |= pend[bj];
+        while (block_any(any != 0ull, flags, vote)) {
+            // --- append flagged values to the per-query candidate slots; a
+            // 16-value group with no flag in any lane of the wave is skipped
+#pragma unroll
+            for (int bj = 0; bj < BB; ++bj) {
+                if (!pend[bj]) continue;
+                const int q = ql[bj];
+#pragma unroll
+                for (int ai = 0; ai < 4; ++ai) {
+                    if (!((pend[bj] >> (ai * 16)) & 0xffffull)) continue;
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const uint64_t bit = 1ull << (ai * 16 + i);
+                        if (pend[bj] & bit) {
+                            const int slot = atomicAdd(&cnt[q], 1);
+                            if (slot < kCap) {
+                                cv[q * CP + slot] = acc[ai][bj][i];
+                                cc[q * CP + slot] =
+                                    (int32_t)(a.col_base + i0 + acc_row(wi, ai, i, lane));
+                                pend[bj] &= ~bit;
+                            }
                         }
                     }
                 }
+            }
             __syncthreads();
             // --- merge: one thread per query inserts its candidates
-            if (tid < kBR) {
-                const int nc = min(cnt[tid], kCap);
-                float *L = lv + tid * LP;
-                int32_t *Lc = lc + tid * LP;
+            for (int q = tid; q < BR; q += NT) {
+                const int nc = min(cnt[q], kCap);
+                if (nc == 0) continue;
+                float *L = lv + q * LP;
+                int32_t *Lc = lc + q * LP;
                 for (int j = 0; j < nc; ++j) {
-                    const float v = cv[tid * CP + j];
-                    const int32_t c = cc[tid * CP + j];
+                    const float v = cv[q * CP + j];
+                    const int32_t c = cc[q * CP + j];
                     if (!beats(v, c, L[KP - 1], Lc[KP - 1])) continue;
                     int pos = KP - 1;
                     while (pos > 0 && beats(v, c, L[pos - 1], Lc[pos - 1])) {
@@ -302,54 +395,81 @@ __global__ __launch_bounds__(256, 1) void gemm_topk_kernel(GemmArgs a) {
                     L[pos] = v;
                     Lc[pos] = c;
                 }
-                cnt[tid] = 0;
-                tauv[tid] = L[KP - 1];
+                cnt[q] = 0;
+                tauv[q] = L[KP - 1];
             }
             __syncthreads();
             // --- overflowed values retry against the raised tau (>=: an equal
             // value of this tile may still win on its lower column)
-            tau0 = tauv[ql0];
-            tau1 = tauv[ql1];
-            if (pend0 | pend1) {
+            any = 0;
 #pragma unroll
-                for (int ai = 0; ai < 4; ++ai)
+            for (int bj = 0; bj < BB; ++bj) {
+                if (!pend[bj]) continue;
+                const float tau = tauv[ql[bj]];
+#pragma unroll
+                for (int ai = 0; ai < 4; ++ai) {
+                    if (!((pend[bj] >> (ai * 16)) & 0xffffull)) continue;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
                         const uint64_t bit = 1ull << (ai * 16 + i);
-                        if ((pend0 & bit) && !(acc[ai][0][i] >= tau0)) pend0 &= ~bit;
-                        if ((pend1 & bit) && !(acc[ai][1][i] >= tau1)) pend1 &= ~bit;
+                        if ((pend[bj] & bit) && !(acc[ai][bj][i] >= tau)) pend[bj] &= ~bit;
                     }
+                }
+                any |= pend[bj];
             }
         }
     }
     __syncthreads();
-    if (tid < kBR && r0 + tid < a.M) {
-        const int64_t base = ((r0 + tid) * a.S + split) * (int64_t)KP;
+    for (int q = tid; q < BR; q += NT) {
+        if (r0 + q >= a.M) continue;
+        const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
         for (int j = 0; j < KP; ++j) {
-            a.part_v[base + j] = lv[tid * LP + j];
-            a.part_c[base + j] = lc[tid * LP + j];
+            a.part_v[base + j] = lv[q * LP + j];
+            a.part_c[base + j] = lc[q * LP + j];
         }
     }
 }
 
-static int launch_common(const GemmArgs &a, const void *fn, size_t lds, hipStream_t st) {
-    int64_t items = a.R * a.S;
+static int launch_common(const GemmArgs &a, const void *fn, size_t lds, int nt, hipStream_t st) {
+    int64_t items = (int64_t)a.ngroups * a.qg * a.S;
     if (items == 0) return RM_OK;
     int64_t G = cdiv(items, 8) * 8;
     RM_CHECK(G < (int64_t)1 << 31, RM_ERR_SHAPE, "grid too large (%lld workgroups)", (long long)G);
+    RM_CHECK(lds <= 160 * 1024, RM_ERR_SHAPE, "fused kernel needs %zu B of LDS", lds);
     RM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     void *args[] = {(void *)&a};
-    RM_HIP(hipLaunchKernel(fn, dim3((unsigned)G), dim3(kThreads), args, lds, st));
+    RM_HIP(hipLaunchKernel(fn, dim3((unsigned)G), dim3(nt), args, lds, st));
     return RM_OK;
 }
 
+// Kernel shapes: KERN_W4 = 4 waves with 128 x 128 wave tiles, KERN_W8 = 8
+// waves (two per SIMD) with 128 x 64 wave tiles -- both 256 x 256 tiles;
+// lists longer than 32 use 4 waves x 128 x 64 (128 queries per tile).
+int g_kernel_shape = KERN_W8;
+
 int launch_gemm_topk(const GemmArgs &a, int elem, hipStream_t st) {
-    const void *fn = elem == 0 ? (const void *)gemm_topk_kernel<0> : (const void *)gemm_topk_kernel<1>;
-    return launch_common(a, fn, topk_lds_bytes(a.KP), st);
+    const void *fn;
+    int nt = 256;
+    if (rows_per_tile(a.KP) == 128)
+        fn = elem == 0 ? (const void *)gemm_topk_kernel<0, 2, 2> : (const void *)gemm_topk_kernel<1, 2, 2>;
+    else if (g_kernel_shape == KERN_W4)
+        fn = elem == 0 ? (const void *)gemm_topk_kernel<0, 2, 4> : (const void *)gemm_topk_kernel<1, 2, 4>;
+    else {
+        fn = elem == 0 ? (const void *)gemm_topk_kernel<0, 4, 2> : (const void *)gemm_topk_kernel<1, 4, 2>;
+        nt = 512;
+    }
+    return launch_common(a, fn, topk_lds_bytes(a.KP), nt, st);
 }
 int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st) {
-    const void *fn = elem == 0 ? (const void *)gemm_store_kernel<0> : (const void *)gemm_store_kernel<1>;
-    return launch_common(a, fn, store_lds_bytes(), st);
+    const void *fn;
+    int nt = 256;
+    if (g_kernel_shape == KERN_W4)
+        fn = elem == 0 ? (const void *)gemm_store_kernel<0, 2, 4> : (const void *)gemm_store_kernel<1, 2, 4>;
+    else {
+        fn = elem == 0 ? (const void *)gemm_store_kernel<0, 4, 2> : (const void *)gemm_store_kernel<1, 4, 2>;
+        nt = 512;
+    }
+    return launch_common(a, fn, store_lds_bytes(), nt, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -408,21 +528,65 @@ __global__ __launch_bounds__(64) void merge_kernel(const float *__restrict__ pv,
 
     if (mode == RM_MODE_FP8_RESCORE) {
         const float *qr = q32 + r * ldq32;
-        for (int t = lane; t < K; t += 64) qs[t] = qr[t];
-        __syncthreads();
-        const int C = (K + 63) / 64;
-        for (int t = 0; t < KP; ++t) {
-            const int32_t c = mc[t];
-            if (c == INT_MAX) { if (lane == 0) sc[t] = -INFINITY; continue; }
-            const uint8_t *col = db + (int64_t)c * ldb;
-            float acc = 0.0f;
-            for (int e = lane * C; e < (lane + 1) * C && e < K; ++e) {
-                const float p = qs[e] * e4m3_to_f32(col[e]);
-                acc = acc + p;
-            }
+        const int C = (K + 63) / 64;  // elements per lane: [lane*C, lane*C + C)
+        const int e_lo = lane * C;
+        if (C % 16 == 0 && K % 64 == 0) {
+            // four candidates per round, 16-byte pieces of each column slice
+            // loaded before use; per candidate the lane still adds its
+            // products in increasing element order
+            constexpr int RG = 4;
+            for (int t0 = 0; t0 < KP; t0 += RG) {
+                float acc[RG];
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
-            if (lane == 0) sc[t] = acc;
+                for (int u = 0; u < RG; ++u) acc[u] = 0.0f;
+                for (int e0 = 0; e0 < C; e0 += 16) {
+                    uint4 cv[RG];
+#pragma unroll
+                    for (int u = 0; u < RG; ++u) {
+                        const int32_t c = (t0 + u < KP) ? mc[t0 + u] : INT_MAX;
+                        cv[u] = c == INT_MAX ? make_uint4(0, 0, 0, 0)
+                                             : *(const uint4 *)(db + (int64_t)c * ldb + e_lo + e0);
+                    }
+                    float qv[16];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float4 x = *(const float4 *)(qr + e_lo + e0 + 4 * j);
+                        qv[4 * j] = x.x; qv[4 * j + 1] = x.y; qv[4 * j + 2] = x.z; qv[4 * j + 3] = x.w;
+                    }
+#pragma unroll
+                    for (int u = 0; u < RG; ++u) {
+                        const uint32_t w[4] = {cv[u].x, cv[u].y, cv[u].z, cv[u].w};
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) {
+                            const float p = qv[j] * e4m3_to_f32((w[j >> 2] >> (8 * (j & 3))) & 0xffu);
+                            acc[u] = acc[u] + p;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < RG; ++u) {
+                    float v = acc[u];
+#pragma unroll
+                    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+                    if (lane == 0 && t0 + u < KP) sc[t0 + u] = (mc[t0 + u] == INT_MAX) ? -INFINITY : v;
+                }
+            }
+        } else {
+            for (int t = lane; t < K; t += 64) qs[t] = qr[t];
+            __syncthreads();
+            for (int t = 0; t < KP; ++t) {
+                const int32_t c = mc[t];
+                if (c == INT_MAX) { if (lane == 0) sc[t] = -INFINITY; continue; }
+                const uint8_t *col = db + (int64_t)c * ldb;
+                float acc = 0.0f;
+                for (int e = e_lo; e < e_lo + C && e < K; ++e) {
+                    const float p = qs[e] * e4m3_to_f32(col[e]);
+                    acc = acc + p;
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
+                if (lane == 0) sc[t] = acc;
+            }
         }
         __syncthreads();
         if (lane < KP) {

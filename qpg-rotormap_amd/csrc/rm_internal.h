@@ -33,22 +33,29 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---- fused GEMM + top-k tile geometry (shared by host planning and kernels)
 constexpr int kBI = 256;       // index columns per tile
-constexpr int kBR = 128;       // reads per tile
 constexpr int kBK = 64;        // K bytes per LDS stage
-constexpr int kRow = 80;       // LDS row pitch in bytes (64 + 16 pad)
+#ifndef RM_KROW
+#define RM_KROW 80
+#endif
+constexpr int kRow = RM_KROW;  // LDS row pitch in bytes (64 + 16 pad)
 constexpr int kCap = 32;       // candidate slots per read per merge round
 constexpr int kThreads = 256;  // 4 waves
 constexpr int kMaxKP = 64;     // largest top-k' list
 
-// LDS bytes of the fused kernel for a list length kp
+// Queries per tile of the fused kernel: 256 while the per-query lists fit
+// LDS beside the stage buffers (k' <= 32), else 128.
+inline int rows_per_tile(int kp) { return kp <= 32 ? 256 : 128; }
+// LDS bytes of the fused kernel for a list length kp: two stage buffers (the
+// candidate slots alias them), the lists, tau and the slot counters
 inline size_t topk_lds_bytes(int kp) {
-    size_t stage = 2 * (size_t)(kBI + kBR) * kRow;
-    size_t lists = (size_t)kBR * (kp + 1) * 8;
-    size_t cands = (size_t)kBR * (kCap + 1) * 8;
-    size_t misc = (size_t)kBR * 8 + 64;
-    return stage + lists + cands + misc;
+    const size_t br = (size_t)rows_per_tile(kp);
+    size_t stage = 2 * (kBI + br) * kRow;
+    size_t cands = br * (kCap + 1) * 8;
+    size_t lists = br * (kp + 1) * 8;
+    size_t misc = br * 8 + 16;
+    return (stage > cands ? stage : cands) + lists + misc;
 }
-inline size_t store_lds_bytes() { return 2 * (size_t)(kBI + kBR) * kRow; }
+inline size_t store_lds_bytes() { return 2 * (size_t)(kBI + 256) * kRow; }
 
 struct GemmArgs {
     const uint8_t *db;  // index rows (K contiguous bytes each), row pitch ldb bytes
@@ -59,7 +66,9 @@ struct GemmArgs {
     int64_t N;          // index rows covered by this launch
     int Kb;             // K in bytes
     int S;              // column splits
-    int64_t R;          // row tiles = ceil(M / kBR)
+    int64_t R;          // row tiles = ceil(M / rows per tile)
+    int qg;             // row tiles per XCD group (items ordered group, split, tile)
+    int ngroups;        // ceil(R / qg)
     int64_t tiles;      // column tiles = ceil(N / kBI)
     int64_t col_base;   // global column id of db row 0
     // top-k epilogue
@@ -76,7 +85,9 @@ struct GemmArgs {
     int *work_counter;
 };
 
-// launchers (gemm_topk.hip)
+// launchers (gemm_topk.hip); the 256 x 256 kernel shape is selectable
+constexpr int KERN_W4 = 0, KERN_W8 = 1;
+extern int g_kernel_shape;
 int launch_gemm_topk(const GemmArgs &a, int elem, hipStream_t st);
 int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st);
 int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int ktop,
@@ -123,7 +134,7 @@ struct rm_ctx {
     size_t table_bytes = 0;
     std::map<std::string, rm_buf> ws;
     // autotuned splits: (mode, M bucket) -> S
-    std::map<std::pair<int, int>, int> plan;
+    std::map<std::pair<int, int>, std::pair<int, int>> plan;  // -> (splits, qgroup)
     // kernel timing (rm_prof_enable): event pairs around fused launches
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
@@ -142,9 +153,9 @@ namespace rm {
 int ws_get(rm_ctx *ctx, const char *name, size_t bytes, void **out);
 int table_get(rm_ctx *ctx, int k, int s, int m, const int32_t **out);
 hipStream_t pick_stream(rm_ctx *ctx, void *stream);
-int default_splits(rm_ctx *ctx, int64_t M, int64_t N);
+int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp);
 // the whole fused query path on already-encoded queries
 int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
               const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
-              float *vals, int32_t *locs, int64_t ldo, int S, hipStream_t st);
+              float *vals, int32_t *locs, int64_t ldo, int S, int qg, hipStream_t st);
 }  // namespace rm

@@ -129,10 +129,11 @@ class TopKEngine:
         """Time split counts per batch-size bucket; the context keeps the best."""
         b = np.ascontiguousarray(bucket_rows, dtype=np.int64)
         best = np.zeros(len(b), dtype=np.int32)
+        qg = np.zeros(len(b), dtype=np.int32)
         ms = np.zeros(len(b), dtype=np.float32)
         L.check(L.lib().rm_autotune(L.ctx(self.device), self.index.handle, L.ptr(b), len(b),
-                                    self.mode, L.ptr(best), L.ptr(ms)))
-        return best, ms
+                                    self.mode, L.ptr(best), L.ptr(qg), L.ptr(ms)))
+        return best, qg, ms
 
 
 def topk_flow(source, eng, re):

@@ -93,17 +93,17 @@ def test_gemm_chunk_fp16_vs_fp64(torch_gpu):
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-6
 
 
-def fused(torch, eng, Q, Q32=None, splits=None):
+def fused(torch, eng, Q, Q32=None, splits=None, qg=0):
     M = Q.shape[0]
     mode = eng.mode
     if splits is not None:
-        L.check(R.lib().rm_plan_set(L.ctx(0), M, mode, splits))
+        L.check(R.lib().rm_plan_set(L.ctx(0), M, mode, splits, qg))
     dq = torch.from_numpy(Q).cuda()
     d32 = torch.from_numpy(Q32).cuda() if Q32 is not None else None
     v, l = eng.batch_gemm_topk(dq, d32)
     torch.cuda.synchronize()
     if splits is not None:
-        L.check(R.lib().rm_plan_set(L.ctx(0), M, mode, 0))
+        L.check(R.lib().rm_plan_set(L.ctx(0), M, mode, 0, 0))
     return v.cpu().numpy(), l.cpu().numpy()
 
 
@@ -121,9 +121,10 @@ def test_fused_topk_bitexact(torch_gpu, M, N):
     for S in sorted({1, 2, 3, tiles}):
         if S > tiles:
             continue
-        v, l = fused(torch, eng, Q, splits=S)
-        np.testing.assert_array_equal(l, el, err_msg=f"locs, splits={S}")
-        np.testing.assert_array_equal(v, ev, err_msg=f"vals, splits={S}")
+        for qg in (0, 2, 3):  # query tiles grouped per XCD (0 = all)
+            v, l = fused(torch, eng, Q, splits=S, qg=qg)
+            np.testing.assert_array_equal(l, el, err_msg=f"locs, splits={S}, qg={qg}")
+            np.testing.assert_array_equal(v, ev, err_msg=f"vals, splits={S}, qg={qg}")
 
 
 def test_fused_topk_ties_and_k64(torch_gpu):
