@@ -150,8 +150,8 @@ int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes
         ctx->prof_ev.push_back({e0, e1});
         ctx->prof_flop += 2.0 * (double)M * (double)K * (double)idx->N;
     }
-    return launch_merge((const float *)pv, (const int32_t *)pc, S, KP, M, ktop, mode, Q32, ldq32,
-                        idx->b8, K, (int)K, vals, locs, ldo, nullptr, st);
+    return launch_merge((const float *)pv, (const int32_t *)pc, S, KP, M, idx->N, ktop, mode, Q32,
+                        ldq32, idx->b8, K, (int)K, vals, locs, ldo, st);
 }
 
 static int check_encoder(int k, int s, int m, int c, int normalize) {

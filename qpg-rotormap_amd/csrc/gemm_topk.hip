@@ -93,10 +93,17 @@ __device__ __forceinline__ void g_load(StageRegs<WR, BB> &st, const TileSrc &ts,
     }
 }
 
+// physical 16-byte chunk of logical chunk c in row `row` of a stage buffer
+__device__ __forceinline__ int phys_chunk(int row, int c) {
+    if constexpr (kSwz == 1) return c ^ ((row >> 2) & 3);
+    else if constexpr (kSwz == 2) return c ^ ((row >> 1) & 3);
+    else return c;
+}
+
 template <int WR, int BB>
 __device__ __forceinline__ void s_store(const StageRegs<WR, BB> &st, uint8_t *sb, int tid) {
-    constexpr int RS = Tile<WR, BB>::NT / 4;
-    uint8_t *p = sb + (tid >> 2) * kRow + (tid & 3) * 16;  // chunk j adds RS rows
+    constexpr int RS = Tile<WR, BB>::NT / 4;  // a multiple of 16: same swizzle per chunk j
+    uint8_t *p = sb + (tid >> 2) * kRow + phys_chunk(tid >> 2, tid & 3) * 16;
 #pragma unroll
     for (int j = 0; j < Tile<WR, BB>::CHUNKS; ++j) *(uint4 *)(p + j * RS * kRow) = st.v[j];
 }
@@ -119,13 +126,15 @@ struct Frags {
     uint4 b[BB][2];
 };
 
+// row = stage buffer + (32-aligned row base + lane % 32) * kRow: the swizzle
+// depends on lane % 32 only
 template <int ELEM>
 __device__ __forceinline__ void ld_pair(uint4 (&d)[2], const uint8_t *row, int lane) {
-    const int hl = lane >> 5;
-    const int o0 = ELEM == 0 ? hl * 32 : hl * 16;            // first 16 bytes
-    const int o1 = ELEM == 0 ? hl * 32 + 16 : 32 + hl * 16;  // second 16 bytes
-    d[0] = *(const uint4 *)(row + o0);
-    d[1] = *(const uint4 *)(row + o1);
+    const int hl = lane >> 5, rl = lane & 31;
+    const int c0 = ELEM == 0 ? 2 * hl : hl;          // first 16 bytes
+    const int c1 = ELEM == 0 ? 2 * hl + 1 : 2 + hl;  // second 16 bytes
+    d[0] = *(const uint4 *)(row + phys_chunk(rl, c0) * 16);
+    d[1] = *(const uint4 *)(row + phys_chunk(rl, c1) * 16);
 }
 
 template <int ELEM, int BB>
@@ -278,23 +287,50 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_store_kernel(GemmArgs a) {
     }
 }
 
+// Merge every query's candidate slots into its sorted list (one thread per
+// query), reset the slot counters and raise tau to the list's k'-th value.
+__device__ __forceinline__ void merge_slots(float *lv, int32_t *lc, const float *cv,
+                                            const int32_t *cc, float *tauv, int *cnt, int BR,
+                                            int KP, int LP, int CP, int capd, int tid, int nt) {
+    for (int q = tid; q < BR; q += nt) {
+        const int nc = min(cnt[q], capd);
+        if (nc == 0) continue;
+        float *L = lv + q * LP;
+        int32_t *Lc = lc + q * LP;
+        for (int j = 0; j < nc; ++j) {
+            const float v = cv[q * CP + j];
+            const int32_t c = cc[q * CP + j];
+            if (!beats(v, c, L[KP - 1], Lc[KP - 1])) continue;
+            int pos = KP - 1;
+            while (pos > 0 && beats(v, c, L[pos - 1], Lc[pos - 1])) {
+                L[pos] = L[pos - 1];
+                Lc[pos] = Lc[pos - 1];
+                --pos;
+            }
+            L[pos] = v;
+            Lc[pos] = c;
+        }
+        cnt[q] = 0;
+        tauv[q] = L[KP - 1];
+    }
+}
+
 template <int ELEM, int WR, int BB>
 __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
     constexpr int BR = Tile<WR, BB>::BR;
     constexpr int NT = Tile<WR, BB>::NT;
     extern __shared__ __align__(16) uint8_t smem[];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid & 1, wr = wid >> 1;
-    const int KP = a.KP, LP = KP + 1, CP = kCap + 1;
+    const int KP = a.KP, LP = KP + 1, capd = a.capd, CP = capd + 1;
+    static_assert(2 * Tile<WR, BB>::STAGE_BYTES == stage_pair_bytes(BR), "stage layout");
     uint8_t *sb0 = smem, *sb1 = smem + Tile<WR, BB>::STAGE_BYTES;
-    // candidate slots alias the stage buffers: they are used only between a
-    // tile's last MFMA and the next tile's first stage store
-    float *cv = (float *)smem;                              // BR x (kCap+1)
-    int32_t *cc = (int32_t *)(cv + BR * CP);
-    float *lv = (float *)(smem + 2 * Tile<WR, BB>::STAGE_BYTES); // BR x (KP+1)
+    float *lv = (float *)(smem + list_offset(BR));       // BR x (KP+1)
     int32_t *lc = (int32_t *)(lv + BR * LP);
-    float *tauv = (float *)(lc + BR * LP);                  // BR
-    int *cnt = (int *)(tauv + BR);                          // BR
-    int *flags = cnt + BR;                                  // 3 vote slots
+    float *cv = (float *)(smem + cand_offset(BR, KP));   // BR x (capd+1)
+    int32_t *cc = (int32_t *)(cv + BR * CP);
+    float *tauv = (float *)(cc + BR * CP);               // BR
+    int *cnt = (int *)(tauv + BR);                       // BR
+    int *flags = cnt + BR;                               // 3 vote slots
 
     int64_t rt;
     int split;
@@ -315,11 +351,16 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
         ql[bj] = wr * 32 * BB + bj * 32 + (lane & 31);  // this lane's queries
         qv[bj] = r0 + ql[bj] < a.M;
     }
+    // accumulator (ai, i) of this lane sits at tile column rowbase + RC(ai, i)
+    const int rowbase = wi * 128 + 4 * (lane >> 5);
+#define RC(ai, i) ((ai) * 32 + 8 * ((i) >> 2) + ((i) & 3))
     v16f acc[4][BB];
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
         tile_product<ELEM, WR, BB>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
-        const bool full = i0 + kBI <= a.N;
+        // columns of this tile valid for this lane: RC(ai, i) < lim
+        const int lim = (int)min<int64_t>(a.N - i0 - rowbase, (int64_t)kBI);
+        const int32_t colbase = (int32_t)(a.col_base + i0 + rowbase);
 
         // --- filter: a query's values are compared one by one only when its
         // tile maximum beats tau.  Strictly above tau: a later column can
@@ -333,26 +374,22 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
 #pragma unroll
             for (int ai = 0; ai < 4; ++ai)
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const bool ok = full || (i0 + acc_row(wi, ai, i, lane) < a.N);
-                    mx = fmaxf(mx, ok ? acc[ai][bj][i] : -INFINITY);
-                }
+                for (int i = 0; i < 16; ++i)
+                    if (RC(ai, i) < lim) mx = fmaxf(mx, acc[ai][bj][i]);
             if (qv[bj] && mx > tau) {
 #pragma unroll
                 for (int ai = 0; ai < 4; ++ai)
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        const bool ok = full || (i0 + acc_row(wi, ai, i, lane) < a.N);
-                        if (ok && acc[ai][bj][i] > tau) pend[bj] |= 1ull << (ai * 16 + i);
-                    }
+                    for (int i = 0; i < 16; ++i)
+                        if (RC(ai, i) < lim && acc[ai][bj][i] > tau) pend[bj] |= 1ull << (ai * 16 + i);
             }
         }
         uint64_t any = 0;
 #pragma unroll
         for (int bj = 0; bj < BB; ++bj) any |= pend[bj];
         while (block_any(any != 0ull, flags, vote)) {
-            // --- append flagged values to the per-query candidate slots; a
-            // 16-value group with no flag in any lane of the wave is skipped
+            // --- append flagged values to the query's slots; a 16-value
+            // group with no flag in any lane of the wave is skipped
 #pragma unroll
             for (int bj = 0; bj < BB; ++bj) {
                 if (!pend[bj]) continue;
@@ -365,39 +402,22 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
                         const uint64_t bit = 1ull << (ai * 16 + i);
                         if (pend[bj] & bit) {
                             const int slot = atomicAdd(&cnt[q], 1);
-                            if (slot < kCap) {
+                            if (slot < capd) {
                                 cv[q * CP + slot] = acc[ai][bj][i];
-                                cc[q * CP + slot] =
-                                    (int32_t)(a.col_base + i0 + acc_row(wi, ai, i, lane));
+                                cc[q * CP + slot] = colbase + RC(ai, i);
                                 pend[bj] &= ~bit;
                             }
                         }
                     }
                 }
             }
-            __syncthreads();
-            // --- merge: one thread per query inserts its candidates
-            for (int q = tid; q < BR; q += NT) {
-                const int nc = min(cnt[q], kCap);
-                if (nc == 0) continue;
-                float *L = lv + q * LP;
-                int32_t *Lc = lc + q * LP;
-                for (int j = 0; j < nc; ++j) {
-                    const float v = cv[q * CP + j];
-                    const int32_t c = cc[q * CP + j];
-                    if (!beats(v, c, L[KP - 1], Lc[KP - 1])) continue;
-                    int pos = KP - 1;
-                    while (pos > 0 && beats(v, c, L[pos - 1], Lc[pos - 1])) {
-                        L[pos] = L[pos - 1];
-                        Lc[pos] = Lc[pos - 1];
-                        --pos;
-                    }
-                    L[pos] = v;
-                    Lc[pos] = c;
-                }
-                cnt[q] = 0;
-                tauv[q] = L[KP - 1];
-            }
+            any = 0;
+#pragma unroll
+            for (int bj = 0; bj < BB; ++bj) any |= pend[bj];
+            // slots of every query still hold what they took: merge only when
+            // some value found its query's slots full
+            if (!block_any(any != 0ull, flags, vote)) break;
+            merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, LP, CP, capd, tid, NT);
             __syncthreads();
             // --- overflowed values retry against the raised tau (>=: an equal
             // value of this tile may still win on its lower column)
@@ -419,6 +439,9 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
             }
         }
     }
+#undef RC
+    __syncthreads();
+    merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, LP, CP, capd, tid, NT);
     __syncthreads();
     for (int q = tid; q < BR; q += NT) {
         if (r0 + q >= a.M) continue;
@@ -447,7 +470,10 @@ static int launch_common(const GemmArgs &a, const void *fn, size_t lds, int nt, 
 // lists longer than 32 use 4 waves x 128 x 64 (128 queries per tile).
 int g_kernel_shape = KERN_W8;
 
-int launch_gemm_topk(const GemmArgs &a, int elem, hipStream_t st) {
+int launch_gemm_topk(const GemmArgs &a0, int elem, hipStream_t st) {
+    GemmArgs a = a0;
+    a.capd = cand_slots(rows_per_tile(a.KP), a.KP);
+    RM_CHECK(a.capd >= 1, RM_ERR_SHAPE, "no LDS left for candidate slots (k' = %d)", a.KP);
     const void *fn;
     int nt = 256;
     if (rows_per_tile(a.KP) == 128)
@@ -483,7 +509,7 @@ int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void merge_kernel(const float *__restrict__ pv,
                                                    const int32_t *__restrict__ pc, int S, int KP,
-                                                   int64_t M, int ktop, int mode,
+                                                   int64_t M, int64_t N, int ktop, int mode,
                                                    const float *__restrict__ q32, int64_t ldq32,
                                                    const uint8_t *__restrict__ db, int64_t ldb, int K,
                                                    float *__restrict__ vals,
@@ -521,7 +547,9 @@ __global__ __launch_bounds__(64) void merge_kernel(const float *__restrict__ pv,
         }
         if (wc == INT_MAX) break;  // only empty slots remain
         if (bu >= 0 && bc == wc) hp[bu]++;  // columns are unique across splits
-        if (lane == 0) { mv[j] = wv; mc[j] = wc; }
+        // a column id outside [0, N) cannot come from the fused kernel; it is
+        // kept out of the gathers below
+        if (lane == 0) { mv[j] = wv; mc[j] = (wc >= 0 && wc < N) ? wc : INT_MAX; }
     }
     for (int t = j + lane; t < KP; t += 64) { mv[t] = -INFINITY; mc[t] = INT_MAX; }
     __syncthreads();
@@ -608,13 +636,13 @@ __global__ __launch_bounds__(64) void merge_kernel(const float *__restrict__ pv,
     }
 }
 
-int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int ktop,
-                 int mode, const float *q32, int64_t ldq32, const uint8_t *db, int64_t ldb, int K,
-                 float *vals, int32_t *locs, int64_t ldo, const int64_t *, hipStream_t st) {
+int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int64_t N,
+                 int ktop, int mode, const float *q32, int64_t ldq32, const uint8_t *db,
+                 int64_t ldb, int K, float *vals, int32_t *locs, int64_t ldo, hipStream_t st) {
     if (M == 0) return RM_OK;
     RM_CHECK(S <= 256, RM_ERR_ARG, "at most 256 column splits (got %d)", S);
     size_t lds = (mode == RM_MODE_FP8_RESCORE) ? (size_t)K * 4 : 0;
-    merge_kernel<<<dim3((unsigned)M), dim3(64), lds, st>>>(part_v, part_c, S, KP, M, ktop, mode,
+    merge_kernel<<<dim3((unsigned)M), dim3(64), lds, st>>>(part_v, part_c, S, KP, M, N, ktop, mode,
                                                            q32, ldq32, db, ldb, K, vals, locs, ldo);
     RM_HIP(hipGetLastError());
     return RM_OK;

@@ -34,27 +34,39 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // ---- fused GEMM + top-k tile geometry (shared by host planning and kernels)
 constexpr int kBI = 256;       // index columns per tile
 constexpr int kBK = 64;        // K bytes per LDS stage
-#ifndef RM_KROW
-#define RM_KROW 80
+// Stage-buffer row layout: RM_LDS_SWZ 0 = 80-byte pitch (16-byte pad);
+// 1 / 2 = 64-byte pitch with the 16-byte chunks XOR-swizzled by
+// (row >> 2) & 3 / (row >> 1) & 3.
+#ifndef RM_LDS_SWZ
+#define RM_LDS_SWZ 1
 #endif
-constexpr int kRow = RM_KROW;  // LDS row pitch in bytes (64 + 16 pad)
-constexpr int kCap = 32;       // candidate slots per read per merge round
+constexpr int kSwz = RM_LDS_SWZ;
+constexpr int kRow = kSwz ? 64 : 80;  // LDS row pitch in bytes
 constexpr int kThreads = 256;  // 4 waves
 constexpr int kMaxKP = 64;     // largest top-k' list
 
 // Queries per tile of the fused kernel: 256 while the per-query lists fit
 // LDS beside the stage buffers (k' <= 32), else 128.
 inline int rows_per_tile(int kp) { return kp <= 32 ? 256 : 128; }
-// LDS bytes of the fused kernel for a list length kp: two stage buffers (the
-// candidate slots alias them), the lists, tau and the slot counters
-inline size_t topk_lds_bytes(int kp) {
-    const size_t br = (size_t)rows_per_tile(kp);
-    size_t stage = 2 * (kBI + br) * kRow;
-    size_t cands = br * (kCap + 1) * 8;
-    size_t lists = br * (kp + 1) * 8;
-    size_t misc = br * 8 + 16;
-    return (stage > cands ? stage : cands) + lists + misc;
+// LDS layout of the fused kernel for br queries per tile and list length kp:
+// [ stage buffers x2 ][ lists br x (kp+1) x 8 ][ candidate slots br x (capd+1)
+// x 8 ][ tau br ][ slot counters br ][ 3 vote flags ].  The candidate slots
+// persist across tiles (merged into the lists only when some query's slots
+// overflow); capd takes what is left of the 160 KiB, at most 64.
+constexpr size_t kLdsMax = 160 * 1024;
+constexpr size_t stage_pair_bytes(int br) { return 2 * (size_t)(kBI + br) * kRow; }
+constexpr size_t list_offset(int br) { return stage_pair_bytes(br); }
+constexpr size_t cand_offset(int br, int kp) { return list_offset(br) + (size_t)br * (kp + 1) * 8; }
+constexpr size_t tail_bytes(int br) { return (size_t)br * 8 + 16; }
+constexpr int cand_slots(int br, int kp) {
+    const size_t used = cand_offset(br, kp) + tail_bytes(br);
+    const long long avail = used >= kLdsMax ? 0 : (long long)((kLdsMax - used) / ((size_t)br * 8)) - 1;
+    return avail > 64 ? 64 : (int)avail;
 }
+constexpr size_t topk_lds_bytes_for(int br, int kp) {
+    return cand_offset(br, kp) + (size_t)br * (cand_slots(br, kp) + 1) * 8 + tail_bytes(br);
+}
+inline size_t topk_lds_bytes(int kp) { return topk_lds_bytes_for(rows_per_tile(kp), kp); }
 inline size_t store_lds_bytes() { return 2 * (size_t)(kBI + 256) * kRow; }
 
 struct GemmArgs {
@@ -75,6 +87,7 @@ struct GemmArgs {
     float *part_v;
     int32_t *part_c;
     int KP;
+    int capd;           // candidate slots per query (cand_slots)
     // store-C epilogue
     void *C;
     int c_dtype;
@@ -90,10 +103,9 @@ constexpr int KERN_W4 = 0, KERN_W8 = 1;
 extern int g_kernel_shape;
 int launch_gemm_topk(const GemmArgs &a, int elem, hipStream_t st);
 int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st);
-int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int ktop,
-                 int mode, const float *q32, int64_t ldq32, const uint8_t *db, int64_t ldb,
-                 int K, float *vals, int32_t *locs, int64_t ldo, const int64_t *row_split_off,
-                 hipStream_t st);
+int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int64_t N,
+                 int ktop, int mode, const float *q32, int64_t ldq32, const uint8_t *db,
+                 int64_t ldb, int K, float *vals, int32_t *locs, int64_t ldo, hipStream_t st);
 int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dtype, int64_t ldc,
                    int64_t M, int64_t wi, int64_t offset, int k, hipStream_t st);
 int launch_transpose_u8(uint8_t *dst, const uint8_t *src, int64_t rows, int64_t cols,
