@@ -64,8 +64,8 @@ def traffic_from_profile():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--reads", type=int, default=65_536)
     ap.add_argument("--fp16-steps", type=int, default=1)
     ap.add_argument("--cpu-reads", type=int, default=0, help="0 = size the CPU sample to ~15 s")

@@ -266,7 +266,7 @@ __global__ void quantize_kernel(uint8_t *__restrict__ dst, const void *__restric
 }
 __global__ void dequantize_kernel(float *__restrict__ dst, const uint8_t *__restrict__ src, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = e4m3_to_f32(src[i]);
+    if (i < n) dst[i] = e4m3_byte_to_f32(src[i], 0);
 }
 __global__ void widen_kernel(uint16_t *__restrict__ dst, const uint8_t *__restrict__ src, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -361,6 +361,7 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
         // columns of this tile valid for this lane: RC(ai, i) < lim
         const int lim = (int)min<int64_t>(a.N - i0 - rowbase, (int64_t)kBI);
         const int32_t colbase = (int32_t)(a.col_base + i0 + rowbase);
+        const bool full = i0 + kBI <= a.N;  // uniform: all columns in range
 
         // --- filter: a query's values are compared one by one only when its
         // tile maximum beats tau.  Strictly above tau: a later column can
@@ -371,11 +372,19 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
             pend[bj] = 0;
             const float tau = tauv[ql[bj]];
             float mx = -INFINITY;
+            if (full) {  // every column of the tile is in range: plain max
 #pragma unroll
-            for (int ai = 0; ai < 4; ++ai)
+                for (int ai = 0; ai < 4; ++ai)
 #pragma unroll
-                for (int i = 0; i < 16; ++i)
-                    if (RC(ai, i) < lim) mx = fmaxf(mx, acc[ai][bj][i]);
+                    for (int i = 0; i < 16; i += 2)
+                        mx = fmaxf(mx, fmaxf(acc[ai][bj][i], acc[ai][bj][i + 1]));
+            } else {
+#pragma unroll
+                for (int ai = 0; ai < 4; ++ai)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i)
+                        if (RC(ai, i) < lim) mx = fmaxf(mx, acc[ai][bj][i]);
+            }
             if (qv[bj] && mx > tau) {
 #pragma unroll
                 for (int ai = 0; ai < 4; ++ai)
@@ -558,46 +567,49 @@ __global__ __launch_bounds__(64) void merge_kernel(const float *__restrict__ pv,
         const float *qr = q32 + r * ldq32;
         const int C = (K + 63) / 64;  // elements per lane: [lane*C, lane*C + C)
         const int e_lo = lane * C;
-        if (C % 16 == 0 && K % 64 == 0) {
-            // four candidates per round, 16-byte pieces of each column slice
-            // loaded before use; per candidate the lane still adds its
-            // products in increasing element order
-            constexpr int RG = 4;
-            for (int t0 = 0; t0 < KP; t0 += RG) {
-                float acc[RG];
+        if (C % 16 == 0 && K % 64 == 0 && KP <= 32) {
+            // all k' candidates at once: per 16-element piece of the lane's
+            // slice, the query piece is loaded once and the 16-byte pieces of
+            // the k' columns are issued before use; each candidate's sum
+            // still runs over the lane's elements in increasing order
+            float acc[32];
 #pragma unroll
-                for (int u = 0; u < RG; ++u) acc[u] = 0.0f;
-                for (int e0 = 0; e0 < C; e0 += 16) {
-                    uint4 cv[RG];
+            for (int t = 0; t < 32; ++t) acc[t] = 0.0f;
+            for (int e0 = 0; e0 < C; e0 += 16) {
+                float qv[16];
 #pragma unroll
-                    for (int u = 0; u < RG; ++u) {
+                for (int j = 0; j < 4; ++j) {
+                    const float4 x = *(const float4 *)(qr + e_lo + e0 + 4 * j);
+                    qv[4 * j] = x.x; qv[4 * j + 1] = x.y; qv[4 * j + 2] = x.z; qv[4 * j + 3] = x.w;
+                }
+#pragma unroll
+                for (int t0 = 0; t0 < 32; t0 += 8) {
+                    if (t0 >= KP) break;
+                    uint4 cv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
                         const int32_t c = (t0 + u < KP) ? mc[t0 + u] : INT_MAX;
                         cv[u] = c == INT_MAX ? make_uint4(0, 0, 0, 0)
                                              : *(const uint4 *)(db + (int64_t)c * ldb + e_lo + e0);
                     }
-                    float qv[16];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const float4 x = *(const float4 *)(qr + e_lo + e0 + 4 * j);
-                        qv[4 * j] = x.x; qv[4 * j + 1] = x.y; qv[4 * j + 2] = x.z; qv[4 * j + 3] = x.w;
-                    }
-#pragma unroll
-                    for (int u = 0; u < RG; ++u) {
+                    for (int u = 0; u < 8; ++u) {
                         const uint32_t w[4] = {cv[u].x, cv[u].y, cv[u].z, cv[u].w};
 #pragma unroll
                         for (int j = 0; j < 16; ++j) {
-                            const float p = qv[j] * e4m3_to_f32((w[j >> 2] >> (8 * (j & 3))) & 0xffu);
-                            acc[u] = acc[u] + p;
+                            const float p = qv[j] * e4m3_byte_to_f32(w[j >> 2], j & 3);
+                            acc[t0 + u] = acc[t0 + u] + p;
                         }
                     }
                 }
+            }
 #pragma unroll
-                for (int u = 0; u < RG; ++u) {
-                    float v = acc[u];
+            for (int t = 0; t < 32; ++t) {
+                if (t >= KP) break;
+                float v = acc[t];
 #pragma unroll
-                    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
-                    if (lane == 0 && t0 + u < KP) sc[t0 + u] = (mc[t0 + u] == INT_MAX) ? -INFINITY : v;
-                }
+                for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+                if (lane == 0) sc[t] = (mc[t] == INT_MAX) ? -INFINITY : v;
             }
         } else {
             for (int t = lane; t < K; t += 64) qs[t] = qr[t];

@@ -40,6 +40,16 @@ __device__ __forceinline__ float e4m3_to_f32(uint32_t b) {
     return (b & 0x80u) ? -v : v;
 }
 
+// byte b (0..3) of w as e4m3 -> fp32 with the hardware converter (exact)
+__device__ __forceinline__ float e4m3_byte_to_f32(uint32_t w, int b) {
+    switch (b) {
+    case 0: return __builtin_amdgcn_cvt_f32_fp8((int)w, 0);
+    case 1: return __builtin_amdgcn_cvt_f32_fp8((int)w, 1);
+    case 2: return __builtin_amdgcn_cvt_f32_fp8((int)w, 2);
+    default: return __builtin_amdgcn_cvt_f32_fp8((int)w, 3);
+    }
+}
+
 __device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
     _Float16 h = (_Float16)f;
     return __builtin_bit_cast(uint16_t, h);

@@ -156,4 +156,7 @@ def test_quantize_e4m3_exhaustive(torch_gpu):
     back = torch.empty(256, dtype=torch.float32, device="cuda")
     R.f8_to_f32(back, torch.arange(256, dtype=torch.uint8, device="cuda"))
     torch.cuda.synchronize()
-    np.testing.assert_array_equal(back.cpu().numpy().view(np.uint32), O.e4m3_table().view(np.uint32))
+    got, exp = back.cpu().numpy(), O.e4m3_table()
+    nan = np.isnan(exp)  # codes 0x7F, 0xFF: NaN (any payload)
+    assert np.all(np.isnan(got[nan]))
+    np.testing.assert_array_equal(got[~nan].view(np.uint32), exp[~nan].view(np.uint32))
