@@ -12,7 +12,11 @@ Workload (one step = one batch):
              packed words resident in HBM before timing
   step       encode -> fused e4m3 GEMM + top-32 -> FP32 rescore -> top-20
 
-python bench.py [--steps K] [--warmup W] [--gpus 1] [--reads M] [--no-cpu]
+serving    Poisson request traces at 50 / 200 / 1000 req/s (reads per request
+           log-uniform on [1, 100000]) through the continuous-batching
+           server: p50 / p99 / p999 latency, offered vs achieved reads/s
+
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--reads M] [--no-cpu] [--no-serve]
 """
 import argparse
 import json
@@ -70,6 +74,11 @@ def main():
     ap.add_argument("--fp16-steps", type=int, default=1)
     ap.add_argument("--cpu-reads", type=int, default=0, help="0 = size the CPU sample to ~15 s")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", default="50,200,1000", help="Poisson arrival rates, req/s")
+    ap.add_argument("--serve-requests", default="200,400,600", help="requests per rate")
+    ap.add_argument("--serve-rows", type=int, default=16384, help="rows per serving launch")
+    ap.add_argument("--serve-streams", type=int, default=2, help="serving launch slots")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
@@ -179,6 +188,33 @@ def main():
         "roofline_fp16": {"achieved": round((flop16 / max(1, nl16)) / (kms16 / max(1, nl16) * 1e-3) / 1e12, 2),
                           "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s"},
     }
+
+    if not args.no_serve:
+        # serving: Poisson arrivals, reads per request log-uniform on
+        # [1, 100000], drawn from the batch's reads held in pinned memory
+        pool = torch.from_numpy(reads.words.view(np.int32)).pin_memory()
+        rates = [float(x) for x in args.serve_rates.split(",")]
+        nreq = [int(x) for x in args.serve_requests.split(",")]
+        serving = {"mode": "fp8_rescore", "rows_per_launch": args.serve_rows,
+                   "launch_slots": args.serve_streams,
+                   "request_reads": "log-uniform on [1, 100000]", "pool_reads": M, "rates": []}
+        with R.Server(eng8.index, re, k=20, kprime=32, mode="fp8_rescore",
+                      max_rows=args.serve_rows, n_streams=args.serve_streams) as srv:
+            for i, (rate, nr) in enumerate(zip(rates, nreq)):
+                t, n = R.poisson_trace(nr, rate, seed=100 + i)
+                res = R.run_trace(srv, pool, t, n, seed=200 + i)
+                res.pop("results"), res.pop("offsets")
+                res = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items()}
+                serving["rates"].append({"rate_req_s": rate, **res})
+                log(f"serve {rate:g} req/s: p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms "
+                    f"p999 {res['p999_ms']} ms; offered {res['offered_reads_s']:.0f} "
+                    f"achieved {res['achieved_reads_s']:.0f} reads/s")
+            serving["server_stats"] = srv.stats()
+        del pool
+        result["serving"] = serving
+        mid = serving["rates"][min(1, len(serving["rates"]) - 1)]
+        result["latency"] = {"rate_req_s": mid["rate_req_s"], "p50_ms": mid["p50_ms"],
+                             "p99_ms": mid["p99_ms"], "p999_ms": mid["p999_ms"]}
 
     if not args.no_cpu:
         sys.path.insert(0, ROOT)

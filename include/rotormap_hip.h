@@ -215,6 +215,10 @@ int rm_server_create(rm_ctx *ctx, const rm_index *idx, int k, int s, int m, int 
 /* Enqueue a request of n fragments (host words, copied).  Returns a request
  * id >= 0 through *req_id. */
 int rm_server_submit(rm_server *srv, const uint32_t *host_words, int64_t n, int64_t *req_id);
+/* Enqueue without copying: the caller keeps host_words valid and unchanged
+ * until rm_server_wait has returned for this id (pinned memory gives the
+ * fastest uploads). */
+int rm_server_submit_ref(rm_server *srv, const uint32_t *host_words, int64_t n, int64_t *req_id);
 /* Poll / wait for a finished request: copies (n x ktop) row-major vals/locs
  * to host buffers.  timeout_ms < 0 waits forever; returns RM_ERR_STATE if
  * not finished within the timeout.  t_* (may be NULL): submit, admit and
@@ -224,6 +228,10 @@ int rm_server_wait(rm_server *srv, int64_t req_id, int timeout_ms, float *host_v
 /* Cancel a request that has not been admitted yet. */
 int rm_server_cancel(rm_server *srv, int64_t req_id);
 int rm_server_stats(rm_server *srv, int64_t *launches, int64_t *rows_done, int64_t *max_inflight);
+/* Seconds on the server's monotonic clock (the clock of the t_* stamps). */
+double rm_server_now(rm_server *srv);
+/* Stop the loop: requests not yet launched end as cancelled (or failed when
+ * some of their rows were launched); launches in flight finish first. */
 int rm_server_destroy(rm_server *srv);
 
 /* ---- synthetic inputs (host, deterministic in their seeds) ----------------

@@ -92,7 +92,7 @@ static int bucket_of(int64_t M) {
 // Tile plan for a batch: (splits, query tiles per XCD group).  The built-in
 // rule fills the CUs with one workgroup each and groups 4 query tiles per
 // XCD; the autotuned plan (rm_autotune / rm_plan_set) overrides per bucket.
-static std::pair<int, int> plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
+std::pair<int, int> plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
     auto it = ctx->plan.find({mode == RM_MODE_FP16 ? 1 : 0, bucket_of(M)});
     int64_t tiles = cdiv(idx->N, kBI);
     if (it != ctx->plan.end())
@@ -104,9 +104,32 @@ static int splits_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int
     return plan_for(ctx, idx, M, mode, kp).first;
 }
 
+static int clamp_splits(const rm_index *idx, int S) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)S, cdiv(idx->N, kBI), (int64_t)256}));
+}
+
+size_t part_bytes(const rm_index *idx, int64_t M, int KP, int S) {
+    return (size_t)M * clamp_splits(idx, S) * KP * 4;
+}
+
 int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
               const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
               float *vals, int32_t *locs, int64_t ldo, int S, int qg, hipStream_t st) {
+    const int KP = (mode == RM_MODE_FP8_RESCORE) ? kprime : ktop;
+    if (M == 0) return RM_OK;
+    const size_t pbytes = part_bytes(idx, M, KP, S);
+    void *pv = nullptr, *pc = nullptr;
+    int rc;
+    if ((rc = ws_get(ctx, "part_v", pbytes, &pv))) return rc;
+    if ((rc = ws_get(ctx, "part_c", pbytes, &pc))) return rc;
+    return topk_core_bufs(ctx, idx, Q, ldq_bytes, elem, Q32, ldq32, M, ktop, kprime, mode, vals,
+                          locs, ldo, S, qg, (float *)pv, (int32_t *)pc, pbytes, st);
+}
+
+int topk_core_bufs(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
+                   const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
+                   float *vals, int32_t *locs, int64_t ldo, int S, int qg, float *pv, int32_t *pc,
+                   size_t pbytes, hipStream_t st) {
     const int KP = (mode == RM_MODE_FP8_RESCORE) ? kprime : ktop;
     const int64_t K = idx->rdim;
     const int esz = elem == 0 ? 1 : 2;
@@ -115,12 +138,10 @@ int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes
              elem == 0 ? "e4m3" : "fp16");
     if (M == 0) return RM_OK;
     const int64_t tiles = cdiv(idx->N, kBI);
-    S = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)S, tiles, (int64_t)256}));
-    void *pv = nullptr, *pc = nullptr;
-    size_t pbytes = (size_t)M * S * KP * 4;
+    S = clamp_splits(idx, S);
+    RM_CHECK(part_bytes(idx, M, KP, S) <= pbytes, RM_ERR_STATE,
+             "split partials need %zu bytes, workspace holds %zu", part_bytes(idx, M, KP, S), pbytes);
     int rc;
-    if ((rc = ws_get(ctx, "part_v", pbytes, &pv))) return rc;
-    if ((rc = ws_get(ctx, "part_c", pbytes, &pc))) return rc;
     GemmArgs a{};
     a.db = db;
     a.ldb = K * esz;
@@ -154,7 +175,7 @@ int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes
                         ldq32, idx->b8, K, (int)K, vals, locs, ldo, st);
 }
 
-static int check_encoder(int k, int s, int m, int c, int normalize) {
+int check_encoder(int k, int s, int m, int c, int normalize) {
     RM_CHECK(c >= 1 && c <= s && s <= 16, RM_ERR_ARG,
              "encoder needs 1 <= c <= s <= 16 (got s = %d, c = %d)", s, c);
     RM_CHECK(s <= k, RM_ERR_ARG, "k = %d < s = %d: no s-mer windows", k, s);
@@ -393,7 +414,11 @@ int64_t rm_index_rdim(const rm_index *idx) { return idx ? idx->rdim : 0; }
 const uint8_t *rm_index_e4m3(const rm_index *idx) { return idx ? idx->b8 : nullptr; }
 const void *rm_index_f16(const rm_index *idx) { return idx ? idx->b16 : nullptr; }
 
-static int check_query(const rm_index *idx, int ktop, int kprime, int mode) {
+}  // extern "C"
+
+namespace rm {
+
+int check_query(const rm_index *idx, int ktop, int kprime, int mode) {
     RM_CHECK(mode == RM_MODE_FP8 || mode == RM_MODE_FP8_RESCORE || mode == RM_MODE_FP16,
              RM_ERR_ARG, "bad mode %d", mode);
     RM_CHECK(ktop >= 1 && ktop <= kMaxKP, RM_ERR_ARG, "ktop must be 1..%d (got %d)", kMaxKP, ktop);
@@ -404,6 +429,10 @@ static int check_query(const rm_index *idx, int ktop, int kprime, int mode) {
     if (mode == RM_MODE_FP16) RM_CHECK(idx->b16, RM_ERR_STATE, "index has no fp16 copy");
     return RM_OK;
 }
+
+}  // namespace rm
+
+extern "C" {
 
 int rm_topk_batch(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq, const float *Q32,
                   int64_t ldq32, int64_t M, int ktop, int kprime, int mode, float *vals,
@@ -427,24 +456,41 @@ int rm_topk_batch(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq, 
 }
 
 // encode n fragments straight into the query layouts the GEMM reads
-static int encode_queries(rm_ctx *ctx, const uint32_t *words, int64_t n, int k, int s, int m,
-                          int c, int mode, int64_t rdim, float *norms, void **q, float **q32,
-                          hipStream_t st) {
+}  // extern "C"
+
+namespace rm {
+
+// encode n fragments into the query layouts the GEMM reads: q (n x rdim,
+// e4m3 or fp16) and, for rescoring, q32 (n x rdim fp32)
+int encode_into(rm_ctx *ctx, const uint32_t *words, int64_t n, int k, int s, int m, int c,
+                int mode, int64_t rdim, float *norms, void *q, float *q32, hipStream_t st) {
     int rc;
     const int32_t *tab;
     if ((rc = table_get(ctx, k, s, m, &tab))) return rc;
     const int elem = mode == RM_MODE_FP16 ? 1 : 0;
+    EncOut o1{q, elem ? RM_F16 : RM_E4M3, rdim, 1}, o2{nullptr, 0, 0, 0};
+    if (mode == RM_MODE_FP8_RESCORE) o2 = EncOut{q32, RM_F32, rdim, 1};
+    return launch_encode(words, n, k, s, m, c, 0, tab, o1, o2, norms, nullptr, st);
+}
+
+static int encode_queries(rm_ctx *ctx, const uint32_t *words, int64_t n, int k, int s, int m,
+                          int c, int mode, int64_t rdim, float *norms, void **q, float **q32,
+                          hipStream_t st) {
+    int rc;
+    const int elem = mode == RM_MODE_FP16 ? 1 : 0;
     if ((rc = ws_get(ctx, "q", (size_t)n * rdim * (elem ? 2 : 1), q))) return rc;
-    EncOut o1{*q, elem ? RM_F16 : RM_E4M3, rdim, 1}, o2{nullptr, 0, 0, 0};
     *q32 = nullptr;
     if (mode == RM_MODE_FP8_RESCORE) {
         void *p;
         if ((rc = ws_get(ctx, "q32", (size_t)n * rdim * 4, &p))) return rc;
         *q32 = (float *)p;
-        o2 = EncOut{p, RM_F32, rdim, 1};
     }
-    return launch_encode(words, n, k, s, m, c, 0, tab, o1, o2, norms, nullptr, st);
+    return encode_into(ctx, words, n, k, s, m, c, mode, rdim, norms, *q, *q32, st);
 }
+
+}  // namespace rm
+
+extern "C" {
 
 int rm_map_batch(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_t n, int k,
                  int s, int m, int c, int ktop, int kprime, int mode, float *vals, int32_t *locs,

@@ -170,4 +170,18 @@ int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp);
 int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
               const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
               float *vals, int32_t *locs, int64_t ldo, int S, int qg, hipStream_t st);
+// the same with caller-owned split partials (pv, pc of pbytes each)
+int topk_core_bufs(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
+                   const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
+                   float *vals, int32_t *locs, int64_t ldo, int S, int qg, float *pv, int32_t *pc,
+                   size_t pbytes, hipStream_t st);
+// bytes of each split-partial array for M rows, list length KP, S splits
+size_t part_bytes(const rm_index *idx, int64_t M, int KP, int S);
+// encode n fragments into caller-owned query buffers (q32 only for rescoring)
+int encode_into(rm_ctx *ctx, const uint32_t *words, int64_t n, int k, int s, int m, int c,
+                int mode, int64_t rdim, float *norms, void *q, float *q32, hipStream_t st);
+int check_encoder(int k, int s, int m, int c, int normalize);
+int check_query(const rm_index *idx, int ktop, int kprime, int mode);
+// (splits, query tiles per XCD group) for a batch of M rows
+std::pair<int, int> plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp);
 }  // namespace rm

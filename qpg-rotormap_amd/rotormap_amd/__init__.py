@@ -13,6 +13,7 @@ from .gemm import f8_to_f32, f32_to_f8, fp8_gemm_hip, launch_seg_rowtopk_merge
 from .genome import Reference, Record, pack_codes, read_fasta, synth_reference, unpack_words, write_fasta
 from .index import IndexData, build_index, load_index, save_index, window_table
 from .reads import ReadSet, even_alloc, parse_read_head, sample_reads
+from .serve import Server, ServedRequest, poisson_trace, run_trace
 from .search import MODES, ResidentIndex, TopKBatch, TopKEngine, count_mapped, topk_flow
 
 __all__ = [n for n in dir() if not n.startswith("_")]

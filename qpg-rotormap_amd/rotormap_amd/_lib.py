@@ -69,6 +69,8 @@ SIGNATURES = {
     "rm_server_create": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, i32, i64, i32, i64,
                                C.POINTER(vp)]),
     "rm_server_submit": (i32, [vp, vp, i64, C.POINTER(i64)]),
+    "rm_server_submit_ref": (i32, [vp, vp, i64, C.POINTER(i64)]),
+    "rm_server_now": (C.c_double, [vp]),
     "rm_server_wait": (i32, [vp, i64, i32, vp, vp, vp, vp, vp]),
     "rm_server_cancel": (i32, [vp, i64]),
     "rm_server_stats": (i32, [vp, vp, vp, vp]),

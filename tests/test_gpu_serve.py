@@ -1,0 +1,106 @@
+"""Serving loop on the GPU: every request's rows come back exactly as a
+one-shot rm_map_batch of the same reads gives them (a row's top-k does not
+depend on the launch it rides in), whether the request is alone, split over
+launches or packed with others; admission, cancellation and error paths;
+a short Poisson trace with latency stamps in order."""
+import numpy as np
+import pytest
+
+import rotormap_amd as R
+from rotormap_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+K, KSTEP = 20_000, 2_000
+
+
+@pytest.fixture(scope="module")
+def setup(torch_gpu):
+    ref = R.synth_reference([600_000, 400_000], gc=0.5, seed=2024)
+    re = R.RopeEncoder(k=K, s=8, m=4, c=4)
+    idx = R.build_index(ref, re, kstep=KSTEP)
+    reads = R.sample_reads(ref, 3000, K, err=0.05, seed=5, draw="random", model=0)
+    return ref, re, idx, reads
+
+
+def oneshot(torch, idx, re, words, mode):
+    eng = R.TopKEngine(idx.embeds8, k=20, kprime=32, mode=mode)
+    n = words.shape[0]
+    dw = torch.from_numpy(words.view(np.int32).reshape(-1).copy()).cuda()
+    v, l = eng.map_words(re, dw, n)
+    torch.cuda.synchronize()
+    return v.cpu().numpy(), l.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", ["fp8_rescore", "fp8", "fp16"])
+def test_server_matches_oneshot(torch_gpu, setup, mode):
+    ref, re, idx, reads = setup
+    v0, l0 = oneshot(torch_gpu, idx, re, reads.words, mode)
+    # 700-row launches: requests split over launches and packed together
+    sizes = [1, 0, 650, 1400, 3, 700, 246]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    assert offs[-1] == 3000
+    with R.Server(R.ResidentIndex(idx.embeds8, fp16=(mode == "fp16")), re, mode=mode,
+                  max_rows=700, n_streams=2) as srv:
+        ids = [srv.submit(reads.words[offs[i]:offs[i + 1]], copy=(i % 2 == 0))
+               for i in range(len(sizes))]
+        out = [srv.wait(rid, n) for rid, n in ids]
+        st = srv.stats()
+    for i, r in enumerate(out):
+        np.testing.assert_array_equal(r.vals, v0[offs[i]:offs[i + 1]])
+        np.testing.assert_array_equal(r.locs, l0[offs[i]:offs[i + 1]])
+        assert r.t_submit <= r.t_admit <= r.t_finish
+    assert st["rows_done"] == 3000 and st["launches"] >= 5
+    print(f"[serve] {mode}: {st}")
+
+
+def test_server_alone_equals_mixed(torch_gpu, setup):
+    ref, re, idx, reads = setup
+    w = reads.words[100:400]
+    with R.Server(idx.embeds8, re, max_rows=4096, n_streams=1) as srv:
+        a = srv.wait(*srv.submit(w))
+    with R.Server(idx.embeds8, re, max_rows=512, n_streams=3) as srv:
+        ids = [srv.submit(reads.words[:100]), srv.submit(w), srv.submit(reads.words[400:1500])]
+        b = [srv.wait(*x) for x in ids][1]
+    np.testing.assert_array_equal(a.vals, b.vals)
+    np.testing.assert_array_equal(a.locs, b.locs)
+
+
+def test_server_admission_and_errors(torch_gpu, setup):
+    ref, re, idx, reads = setup
+    B = R.ResidentIndex(idx.embeds8)
+    with pytest.raises(R.RotorMapError) as e:
+        R.Server(B, re, max_rows=65536, n_streams=2, hbm_budget=1 << 20)
+    assert e.value.code == L.RM_ERR_NOMEM and "bytes available" in str(e.value)
+    with pytest.raises(R.RotorMapArgumentError):
+        R.Server(B, R.RopeEncoder(k=K, s=5, m=1, c=4), max_rows=64)  # rdim 512 != 2048
+    with pytest.raises(R.RotorMapArgumentError):
+        R.Server(B, re, max_rows=0)
+    with R.Server(B, re, max_rows=256, n_streams=1) as srv:
+        with pytest.raises(R.RotorMapArgumentError):
+            srv.wait(12345, 1)
+        with pytest.raises(R.RotorMapArgumentError):
+            srv.submit(reads.words[:3].reshape(-1)[:-1])  # not a multiple of W
+        # a long queue behind one slot: the last request is still queued
+        ids = [srv.submit(reads.words[:2000]) for _ in range(4)]
+        last = srv.submit(reads.words[:10])
+        srv.cancel(last[0])
+        with pytest.raises(R.RotorMapError) as e:
+            srv.wait(*last)
+        assert "cancelled" in str(e.value)
+        for x in ids:
+            assert srv.wait(*x).locs.shape == (2000, 20)
+
+
+def test_server_trace(torch_gpu, setup):
+    ref, re, idx, reads = setup
+    pool = np.ascontiguousarray(reads.words)
+    t, n = R.poisson_trace(40, rate=200.0, seed=3, max_reads=2000)
+    with R.Server(idx.embeds8, re, max_rows=4096, n_streams=2) as srv:
+        res = R.run_trace(srv, pool, t, n, seed=4, keep_results=True)
+    assert res["requests"] == 40 and res["reads"] == int(np.minimum(n, 3000).sum())
+    assert 0 < res["p50_ms"] <= res["p99_ms"] <= res["p999_ms"]
+    v0, l0 = oneshot(torch_gpu, idx, re, reads.words, "fp8_rescore")
+    for r, o, m in zip(res["results"], res["offsets"], np.minimum(n, 3000)):
+        np.testing.assert_array_equal(r.locs, l0[o:o + m])
+    print(f"[serve] trace p50 {res['p50_ms']:.2f} ms p99 {res['p99_ms']:.2f} ms")
