@@ -183,6 +183,12 @@ int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int splits, int qgroup);
 int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, int nbuckets,
                 int mode, int32_t *best_splits, int32_t *best_qgroup, float *best_ms);
 
+/* Launch shape of the fused kernels (no counterpart in the reference): on
+ * (the default), a grid of one workgroup per CU walks the (row tile, split)
+ * items of the launch -- one persistent launch however uneven the rows of
+ * the batches concatenated into it; off, one workgroup per item.  Results
+ * are identical either way. */
+int rm_set_persistent(rm_ctx *ctx, int on);
 /* ---- kernel timing ---------------------------------------------------------
  * When enabled, the context records a pair of HIP events around every launch
  * of the fused GEMM + top-k kernel, on the stream it is launched on.

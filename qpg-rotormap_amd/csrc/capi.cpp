@@ -72,13 +72,14 @@ int table_get(rm_ctx *ctx, int k, int s, int m, const int32_t **out) {
     return RM_OK;
 }
 
-// Column splits for M query rows over N index columns: enough (row tile,
-// split) items for one resident workgroup per CU (the fused kernel holds a
-// CU's LDS), never more splits than column tiles; the autotuned plan
-// overrides per M bucket.
+// Column splits for M query rows over N index columns: as many (row tile,
+// split) items as fit one wave of resident workgroups (one per CU: the fused
+// kernel holds a CU's LDS) -- rounding up would leave a second, mostly idle
+// wave -- never more splits than column tiles; the autotuned plan overrides
+// per M bucket.
 int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp) {
     int64_t R = cdiv(M, rows_per_tile(kp)), tiles = cdiv(N, kBI);
-    int64_t want = cdiv((int64_t)ctx->num_cus, R);
+    int64_t want = std::max<int64_t>(1, (int64_t)ctx->num_cus / R);
     int64_t S = std::max<int64_t>(1, std::min<int64_t>(want, tiles));
     return (int)std::min<int64_t>(S, 256);
 }
@@ -159,6 +160,7 @@ int topk_core_bufs(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_
     a.part_v = (float *)pv;
     a.part_c = (int32_t *)pc;
     a.KP = KP;
+    a.persist_grid = ctx->persistent ? ctx->num_cus : 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->prof) {
         RM_HIP(hipEventCreate(&e0));
@@ -342,6 +344,7 @@ int rm_gemm_chunk(rm_ctx *ctx, void *C, int c_dtype, int64_t ldc, const void *A,
     a.C = C;
     a.c_dtype = c_dtype;
     a.ldc = ldc;
+    a.persist_grid = ctx->persistent ? ctx->num_cus : 0;
     return launch_gemm_store(a, ab_dtype == RM_E4M3 ? 0 : 1, st);
 }
 
@@ -560,7 +563,6 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
     const int elem = mode == RM_MODE_FP16 ? 1 : 0;
     const int64_t rdim = idx->rdim;
     const int esz = elem ? 2 : 1;
-    const int64_t tiles = cdiv(idx->N, kBI);
     const int kp = mode == RM_MODE_FP8_RESCORE ? 32 : 20;
     hipEvent_t e0, e1;
     RM_HIP(hipEventCreate(&e0));
@@ -584,9 +586,16 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
         }
         if (q32 && (rc = launch_dequantize((float *)q32, (const uint8_t *)q, M * rdim, st))) return rc;
         const int64_t R = cdiv(M, rows_per_tile(kp));
+        // time each candidate on a prefix of the index sized to ~4e12 FLOP per
+        // launch (a few ms), at least 256 tiles: the plan depends on the row
+        // count, while the per-column work only scales the time
+        rm_index sub = *idx;
+        const int64_t ncap = std::max<int64_t>(256LL * kBI, (int64_t)(4e12 / (2.0 * M * rdim)));
+        sub.N = std::min<int64_t>(idx->N, ncap);
+        const int64_t stiles = cdiv(sub.N, kBI);
         std::vector<std::pair<int, int>> cands;
-        for (int S = 1; S <= 256 && S <= tiles; S *= 2) {
-            if (R * S > 64LL * ctx->num_cus && S > 1) break;
+        for (int S = 1; S <= 256 && S <= stiles; S *= 2) {
+            if (R * S > 4LL * ctx->num_cus && S > 1) break;
             for (int qg : {0, 8, 4, 2}) {
                 if (qg && qg >= R) continue;
                 cands.push_back({S, qg});
@@ -596,7 +605,7 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
         float bestT = 1e30f;
         for (auto c : cands) {
             auto run = [&]() {
-                return topk_core(ctx, idx, q, rdim * esz, elem, (float *)q32, rdim, M, 20, kp, mode,
+                return topk_core(ctx, &sub, q, rdim * esz, elem, (float *)q32, rdim, M, 20, kp, mode,
                                  (float *)vo, (int32_t *)lo, M, c.first, c.second, st);
             };
             if ((rc = run())) return rc;  // warm-up
@@ -618,6 +627,13 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    return RM_OK;
+}
+
+int rm_set_persistent(rm_ctx *ctx, int on) {
+    RM_CHECK(ctx, RM_ERR_ARG, "NULL context");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->persistent = on != 0;
     return RM_OK;
 }
 

@@ -217,6 +217,16 @@ __device__ __forceinline__ void tile_product(v16f (&acc)[4][BB], const GemmArgs 
     }
 }
 
+// Float <-> int key with the same order (signed int compare): negative
+// floats get their magnitude bits flipped.
+__device__ __forceinline__ int order_key(float f) {
+    const int b = __float_as_int(f);
+    return b >= 0 ? b : b ^ 0x7fffffff;
+}
+__device__ __forceinline__ float order_key_inv(int k) {
+    return __int_as_float(k >= 0 ? k : k ^ 0x7fffffff);
+}
+
 // Block-wide OR of a predicate with one barrier: waves with a true lane set
 // flag slot `k % 3`; thread 0 clears the next slot for a later vote.
 __device__ __forceinline__ bool block_any(bool p, int *flags, int &k) {
@@ -243,8 +253,8 @@ __device__ __forceinline__ int acc_row(int wi, int ai, int i, int lane) {
 // query tile in group): the resident workgroups of an XCD then cover qg query
 // tiles x S_xcd splits, so each query tile and each index tile is fetched
 // into that XCD's L2 once and shared.
-__device__ __forceinline__ bool decode_item(const GemmArgs &a, int64_t &rt, int &split) {
-    const int64_t G = gridDim.x, g = blockIdx.x;
+__device__ __forceinline__ bool decode_item(const GemmArgs &a, int64_t g, int64_t G, int64_t &rt,
+                                            int &split) {
     const int64_t L = (g & 7) * (G >> 3) + (g >> 3);
     const int64_t per = (int64_t)a.qg * a.S;
     const int64_t grp = L / per, rem = L % per;
@@ -254,11 +264,8 @@ __device__ __forceinline__ bool decode_item(const GemmArgs &a, int64_t &rt, int 
 }
 
 template <int ELEM, int WR, int BB>
-__global__ __launch_bounds__(128 * WR, 1) void gemm_store_kernel(GemmArgs a) {
+__device__ __forceinline__ void store_item(const GemmArgs &a, int64_t rt, int split) {
     extern __shared__ __align__(16) uint8_t smem[];
-    int64_t rt;
-    int split;
-    if (!decode_item(a, rt, split)) return;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid & 1, wr = wid >> 1;
     uint8_t *sb0 = smem, *sb1 = smem + Tile<WR, BB>::STAGE_BYTES;
     const int64_t r0 = rt * Tile<WR, BB>::BR;
@@ -284,6 +291,20 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_store_kernel(GemmArgs a) {
                 }
         }
         __syncthreads();  // the next tile's prologue refills the stage buffers
+    }
+}
+
+// Hardware slot `it` walks the padded item range with the grid's stride: a
+// grid of one workgroup per item runs each item once; a persistent grid
+// (a multiple of 8 workgroups, at most one per CU) keeps every workgroup on
+// its XCD's item range, since the stride preserves it % 8.
+template <int ELEM, int WR, int BB>
+__global__ __launch_bounds__(128 * WR, 1) void gemm_store_kernel(GemmArgs a) {
+    for (int64_t it = blockIdx.x; it < a.items8; it += gridDim.x) {
+        int64_t rt;
+        int split;
+        if (!decode_item(a, it, a.items8, rt, split)) continue;
+        store_item<ELEM, WR, BB>(a, rt, split);
     }
 }
 
@@ -316,7 +337,7 @@ __device__ __forceinline__ void merge_slots(float *lv, int32_t *lc, const float 
 }
 
 template <int ELEM, int WR, int BB>
-__global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
+__device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int split) {
     constexpr int BR = Tile<WR, BB>::BR;
     constexpr int NT = Tile<WR, BB>::NT;
     extern __shared__ __align__(16) uint8_t smem[];
@@ -332,9 +353,6 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
     int *cnt = (int *)(tauv + BR);                       // BR
     int *flags = cnt + BR;                               // 3 vote slots
 
-    int64_t rt;
-    int split;
-    if (!decode_item(a, rt, split)) return;
     const int64_t r0 = rt * BR;
     const int64_t t0 = (int64_t)split * a.tiles / a.S, t1 = (int64_t)(split + 1) * a.tiles / a.S;
 
@@ -362,6 +380,47 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
         const int lim = (int)min<int64_t>(a.N - i0 - rowbase, (int64_t)kBI);
         const int32_t colbase = (int32_t)(a.col_base + i0 + rowbase);
         const bool full = i0 + kBI <= a.N;  // uniform: all columns in range
+
+        // --- first tile of the item: seed tau with a lower bound of each
+        // query's k'-th value, so the first tiles do not append every value.
+        // A lane's 64 values of a query form 4 groups of 16 (ai); a group's
+        // second largest has 2 values at or above it, so over the query's 4
+        // lanes (2 per wave, 2 waves) the smallest second maximum has >= 32
+        // values at or above it (the smallest maximum: >= 16).  Values equal
+        // to the bound must stay candidates: tau is the next float below.
+        if (t == t0 && KP <= 32) {
+            int key[BB];
+#pragma unroll
+            for (int bj = 0; bj < BB; ++bj) {
+                float bound = INFINITY;
+#pragma unroll
+                for (int ai = 0; ai < 4; ++ai) {
+                    float m1 = -INFINITY, m2 = -INFINITY;
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const float v = RC(ai, i) < lim ? acc[ai][bj][i] : -INFINITY;
+                        m2 = fmaxf(m2, fminf(m1, v));
+                        m1 = fmaxf(m1, v);
+                    }
+                    bound = fminf(bound, KP <= 16 ? m1 : m2);
+                }
+                key[bj] = order_key(bound);
+            }
+            int *seed = (int *)sb0;  // the stage buffers are idle until the next tile
+            __syncthreads();
+            for (int i = tid; i < BR; i += NT) seed[i] = INT_MAX;
+            __syncthreads();
+#pragma unroll
+            for (int bj = 0; bj < BB; ++bj)
+                if (qv[bj]) atomicMin(&seed[ql[bj]], key[bj]);
+            __syncthreads();
+            for (int i = tid; i < BR; i += NT) {
+                const int k = seed[i];
+                const float f = order_key_inv(k);
+                if (k != INT_MAX && f > -INFINITY) tauv[i] = nextafterf(f, -INFINITY);
+            }
+            __syncthreads();
+        }
 
         // --- filter: a query's values are compared one by one only when its
         // tile maximum beats tau.  Strictly above tau: a later column can
@@ -403,19 +462,22 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
             for (int bj = 0; bj < BB; ++bj) {
                 if (!pend[bj]) continue;
                 const int q = ql[bj];
+                // one atomic reserves a slot per flagged value; the first
+                // capd - slot of them (in (ai, i) order) are written, the
+                // rest stay pending for the retry after the merge
+                int slot = atomicAdd(&cnt[q], __popcll(pend[bj]));
+                if (slot >= capd) continue;
 #pragma unroll
                 for (int ai = 0; ai < 4; ++ai) {
                     if (!((pend[bj] >> (ai * 16)) & 0xffffull)) continue;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
                         const uint64_t bit = 1ull << (ai * 16 + i);
-                        if (pend[bj] & bit) {
-                            const int slot = atomicAdd(&cnt[q], 1);
-                            if (slot < capd) {
-                                cv[q * CP + slot] = acc[ai][bj][i];
-                                cc[q * CP + slot] = colbase + RC(ai, i);
-                                pend[bj] &= ~bit;
-                            }
+                        if ((pend[bj] & bit) && slot < capd) {
+                            cv[q * CP + slot] = acc[ai][bj][i];
+                            cc[q * CP + slot] = colbase + RC(ai, i);
+                            pend[bj] &= ~bit;
+                            ++slot;
                         }
                     }
                 }
@@ -462,10 +524,24 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
     }
 }
 
-static int launch_common(const GemmArgs &a, const void *fn, size_t lds, int nt, hipStream_t st) {
-    int64_t items = (int64_t)a.ngroups * a.qg * a.S;
+template <int ELEM, int WR, int BB>
+__global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
+    for (int64_t it = blockIdx.x; it < a.items8; it += gridDim.x) {
+        int64_t rt;
+        int split;
+        if (!decode_item(a, it, a.items8, rt, split)) continue;
+        topk_item<ELEM, WR, BB>(a, rt, split);
+        __syncthreads();  // the next item re-initialises the lists in LDS
+    }
+}
+
+static int launch_common(const GemmArgs &a0, const void *fn, size_t lds, int nt, hipStream_t st) {
+    GemmArgs a = a0;
+    const int64_t items = (int64_t)a.ngroups * a.qg * a.S;
     if (items == 0) return RM_OK;
-    int64_t G = cdiv(items, 8) * 8;
+    a.items8 = cdiv(items, 8) * 8;
+    int64_t G = a.items8;
+    if (a.persist_grid >= 8) G = std::min<int64_t>(G, a.persist_grid / 8 * 8);
     RM_CHECK(G < (int64_t)1 << 31, RM_ERR_SHAPE, "grid too large (%lld workgroups)", (long long)G);
     RM_CHECK(lds <= 160 * 1024, RM_ERR_SHAPE, "fused kernel needs %zu B of LDS", lds);
     RM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -474,10 +550,9 @@ static int launch_common(const GemmArgs &a, const void *fn, size_t lds, int nt, 
     return RM_OK;
 }
 
-// Kernel shapes: KERN_W4 = 4 waves with 128 x 128 wave tiles, KERN_W8 = 8
-// waves (two per SIMD) with 128 x 64 wave tiles -- both 256 x 256 tiles;
-// lists longer than 32 use 4 waves x 128 x 64 (128 queries per tile).
-int g_kernel_shape = KERN_W8;
+// Kernel shapes: 8 waves (two per SIMD) with 128 x 64 wave tiles over
+// 256 x 256 tiles; lists longer than 32 use 4 waves x 128 x 64 (128 queries
+// per tile).
 
 int launch_gemm_topk(const GemmArgs &a0, int elem, hipStream_t st) {
     GemmArgs a = a0;
@@ -487,8 +562,6 @@ int launch_gemm_topk(const GemmArgs &a0, int elem, hipStream_t st) {
     int nt = 256;
     if (rows_per_tile(a.KP) == 128)
         fn = elem == 0 ? (const void *)gemm_topk_kernel<0, 2, 2> : (const void *)gemm_topk_kernel<1, 2, 2>;
-    else if (g_kernel_shape == KERN_W4)
-        fn = elem == 0 ? (const void *)gemm_topk_kernel<0, 2, 4> : (const void *)gemm_topk_kernel<1, 2, 4>;
     else {
         fn = elem == 0 ? (const void *)gemm_topk_kernel<0, 4, 2> : (const void *)gemm_topk_kernel<1, 4, 2>;
         nt = 512;
@@ -497,14 +570,8 @@ int launch_gemm_topk(const GemmArgs &a0, int elem, hipStream_t st) {
 }
 int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st) {
     const void *fn;
-    int nt = 256;
-    if (g_kernel_shape == KERN_W4)
-        fn = elem == 0 ? (const void *)gemm_store_kernel<0, 2, 4> : (const void *)gemm_store_kernel<1, 2, 4>;
-    else {
-        fn = elem == 0 ? (const void *)gemm_store_kernel<0, 4, 2> : (const void *)gemm_store_kernel<1, 4, 2>;
-        nt = 512;
-    }
-    return launch_common(a, fn, store_lds_bytes(), nt, st);
+    fn = elem == 0 ? (const void *)gemm_store_kernel<0, 4, 2> : (const void *)gemm_store_kernel<1, 4, 2>;
+    return launch_common(a, fn, store_lds_bytes(), 512, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -92,15 +92,15 @@ struct GemmArgs {
     void *C;
     int c_dtype;
     int64_t ldc;
-    // persistent work list (uneven batches): items of (row tile, split)
-    const int32_t *work;  // NULL: grid mapping; else 3 ints per item {rowtile, split, S}
-    int64_t nwork;
-    int *work_counter;
+    // grid: items8 = item count rounded up to a multiple of 8 (set by the
+    // launcher); persist_grid >= 8 caps the grid at that many workgroups
+    // (rounded down to a multiple of 8), each walking items with the grid's
+    // stride; 0 launches one workgroup per item
+    int64_t items8;
+    int persist_grid;
 };
 
-// launchers (gemm_topk.hip); the 256 x 256 kernel shape is selectable
-constexpr int KERN_W4 = 0, KERN_W8 = 1;
-extern int g_kernel_shape;
+// launchers (gemm_topk.hip)
 int launch_gemm_topk(const GemmArgs &a, int elem, hipStream_t st);
 int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st);
 int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int64_t N,
@@ -148,6 +148,7 @@ struct rm_ctx {
     // autotuned splits: (mode, M bucket) -> S
     std::map<std::pair<int, int>, std::pair<int, int>> plan;  // -> (splits, qgroup)
     // kernel timing (rm_prof_enable): event pairs around fused launches
+    bool persistent = true;  // fused launches use a grid of one workgroup per CU
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
     double prof_flop = 0;

@@ -135,7 +135,9 @@ def test_fused_topk_ties_and_k64(torch_gpu):
     B = np.concatenate([base] * 30)  # 1200 columns, each value 30 times
     Q = rand_e4m3((M, K), 0.05, 12)
     C = gemm_c(torch, Q, B)
-    for k in (20, 64):
+    # k <= 32: 256-query tiles; 33..48: 128-query tiles, rank merge; 64:
+    # 128-query tiles, sequential merge
+    for k in (20, 32, 40, 48, 64):
         ev, el = O.topk_rows(C, k)
         eng = R.TopKEngine(B, k=k, mode="fp8")
         for S in (1, 4):
@@ -202,3 +204,25 @@ def test_rowtopk_merge_dropin(torch_gpu):
         ev, el = O.topk_rows(Call, k)
         np.testing.assert_array_equal(Dl.cpu().numpy(), el)
         np.testing.assert_array_equal(Dv.cpu().numpy(), ev)
+
+
+def test_persistent_grid_bitexact(torch_gpu):
+    """One workgroup per CU walking the (row tile, split) items and one
+    workgroup per item give identical top-k, also when each workgroup runs
+    many items (16 row tiles x 128 splits = 2048 items)."""
+    torch = torch_gpu
+    K, k, M, N = 512, 20, 4096, 2 ** 15
+    Q = rand_e4m3((M, K), 0.05, 5)
+    B = rand_e4m3((N, K), 0.05, 6)
+    ev, el = O.topk_rows(gemm_c(torch, Q, B), k)
+    eng = R.TopKEngine(B, k=k, mode="fp8")
+    ctx = L.ctx(0)
+    try:
+        for on in (1, 0):
+            L.check(R.lib().rm_set_persistent(ctx, on))
+            for S in (1, 16, 128):
+                v, l = fused(torch, eng, Q, splits=S, qg=0)
+                np.testing.assert_array_equal(l, el, err_msg=f"locs, persistent={on}, S={S}")
+                np.testing.assert_array_equal(v, ev, err_msg=f"vals, persistent={on}, S={S}")
+    finally:
+        L.check(R.lib().rm_set_persistent(ctx, 1))
