@@ -593,8 +593,11 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
         const int64_t ncap = std::max<int64_t>(256LL * kBI, (int64_t)(4e12 / (2.0 * M * rdim)));
         sub.N = std::min<int64_t>(idx->N, ncap);
         const int64_t stiles = cdiv(sub.N, kBI);
+        // splits from a quarter of one wave of workgroups to four waves
+        const int64_t wave = std::max<int64_t>(1, ctx->num_cus / R);
         std::vector<std::pair<int, int>> cands;
         for (int S = 1; S <= 256 && S <= stiles; S *= 2) {
+            if (4 * S < wave && 2 * S <= std::min<int64_t>(256, stiles)) continue;
             if (R * S > 4LL * ctx->num_cus && S > 1) break;
             for (int qg : {0, 8, 4, 2}) {
                 if (qg && qg >= R) continue;

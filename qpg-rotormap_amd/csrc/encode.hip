@@ -188,9 +188,19 @@ int launch_encode(const uint32_t *words, int64_t n, int k, int s, int m, int c, 
              "encoder needs %zu B of LDS (> 160 KiB): k = %d, D = %d too large", lds, k, D);
     RM_HIP(hipFuncSetAttribute((const void *)rope_encode_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    rope_encode_kernel<<<dim3((unsigned)n), dim3(256), lds, st>>>(
-        words, n, k, s, m, c, normalize, (const int2 *)table, o1, o2, norms, counts);
-    RM_HIP(hipGetLastError());
+    // one workgroup of 256 per fragment: launches of < 2^32 work-items
+    const int64_t chunk = kMaxWorkItems / 256;
+    auto esz = [](int dt) { return dt == RM_F32 ? 4 : dt == RM_F16 ? 2 : 1; };
+    for (int64_t f0 = 0; f0 < n; f0 += chunk) {
+        const int64_t nf = std::min(chunk, n - f0);
+        EncOut a1 = o1, a2 = o2;
+        a1.p = (uint8_t *)o1.p + f0 * o1.sf * esz(o1.dtype);
+        if (o2.p) a2.p = (uint8_t *)o2.p + f0 * o2.sf * esz(o2.dtype);
+        rope_encode_kernel<<<dim3((unsigned)nf), dim3(256), lds, st>>>(
+            words + f0 * W, nf, k, s, m, c, normalize, (const int2 *)table, a1, a2,
+            norms ? norms + f0 * m : nullptr, counts ? counts + f0 * ((int64_t)1 << (2 * c)) : nullptr);
+        RM_HIP(hipGetLastError());
+    }
     return RM_OK;
 }
 
@@ -208,16 +218,17 @@ __device__ __forceinline__ uint32_t lanerev32(uint32_t x) {
 // a zero tail (the aligned shift pushes the padding out).
 __global__ void revcomp_kernel(uint32_t *__restrict__ rc, const uint32_t *__restrict__ w,
                                int64_t n, int W, uint32_t sh) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n * W) return;
-    int64_t fr = g / W;
-    int p = (int)(g % W);
-    int i = W - 1 - p;  // aligned source word
-    const uint32_t *fw = w + fr * W;
-    uint32_t x = ~fw[i];
-    uint32_t v = x;
-    if (sh) v = (x << sh) | (i > 0 ? (~fw[i - 1] >> (32u - sh)) : 0u);
-    rc[g] = lanerev32(v);
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n * W;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t fr = g / W;
+        const int p = (int)(g % W);
+        const int i = W - 1 - p;  // aligned source word
+        const uint32_t *fw = w + fr * W;
+        const uint32_t x = ~fw[i];
+        uint32_t v = x;
+        if (sh) v = (x << sh) | (i > 0 ? (~fw[i - 1] >> (32u - sh)) : 0u);
+        rc[g] = lanerev32(v);
+    }
 }
 
 int launch_revcomp(uint32_t *rc, const uint32_t *w, int64_t n, int k, hipStream_t st) {
@@ -225,7 +236,7 @@ int launch_revcomp(uint32_t *rc, const uint32_t *w, int64_t n, int k, hipStream_
     int64_t tot = n * W;
     if (tot == 0) return RM_OK;
     uint32_t sh = (uint32_t)(32 * W - 2 * k);
-    revcomp_kernel<<<dim3((unsigned)cdiv(tot, 256)), dim3(256), 0, st>>>(rc, w, n, W, sh);
+    revcomp_kernel<<<dim3(stride_grid(tot, 256)), dim3(256), 0, st>>>(rc, w, n, W, sh);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }
@@ -234,17 +245,18 @@ int launch_revcomp(uint32_t *rc, const uint32_t *w, int64_t n, int k, hipStream_
 // from bit 0 with the tail masked to zero.
 __global__ void extract_kernel(uint32_t *__restrict__ out, const uint32_t *__restrict__ rec,
                                const int64_t *__restrict__ starts, int64_t n, int k, int W) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n * W) return;
-    int64_t i = g / W;
-    int w = (int)(g % W);
-    int64_t b = 2 * starts[i] + 32 * (int64_t)w;
-    int64_t q = b >> 5;
-    uint32_t sh = (uint32_t)(b & 31);
-    uint32_t v = sh ? ((rec[q] >> sh) | (rec[q + 1] << (32u - sh))) : rec[q];
-    int r = k & 15;
-    if (w == W - 1 && r) v &= (1u << (2 * r)) - 1u;
-    out[g] = v;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n * W;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = g / W;
+        const int w = (int)(g % W);
+        const int64_t b = 2 * starts[i] + 32 * (int64_t)w;
+        const int64_t q = b >> 5;
+        const uint32_t sh = (uint32_t)(b & 31);
+        uint32_t v = sh ? ((rec[q] >> sh) | (rec[q + 1] << (32u - sh))) : rec[q];
+        const int r = k & 15;
+        if (w == W - 1 && r) v &= (1u << (2 * r)) - 1u;
+        out[g] = v;
+    }
 }
 
 int launch_extract(uint32_t *out, const uint32_t *rec, const int64_t *starts, int64_t n, int k,
@@ -252,42 +264,46 @@ int launch_extract(uint32_t *out, const uint32_t *rec, const int64_t *starts, in
     int W = (k + 15) / 16;
     int64_t tot = n * W;
     if (tot == 0) return RM_OK;
-    extract_kernel<<<dim3((unsigned)cdiv(tot, 256)), dim3(256), 0, st>>>(out, rec, starts, n, k, W);
+    extract_kernel<<<dim3(stride_grid(tot, 256)), dim3(256), 0, st>>>(out, rec, starts, n, k, W);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }
 
 __global__ void quantize_kernel(uint8_t *__restrict__ dst, const void *__restrict__ src, int sd,
                                 int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float v = (sd == RM_F32) ? ((const float *)src)[i] : f16_bits_to_f32(((const uint16_t *)src)[i]);
-    dst[i] = f32_to_e4m3(v);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = (sd == RM_F32) ? ((const float *)src)[i]
+                                       : f16_bits_to_f32(((const uint16_t *)src)[i]);
+        dst[i] = f32_to_e4m3(v);
+    }
 }
 __global__ void dequantize_kernel(float *__restrict__ dst, const uint8_t *__restrict__ src, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = e4m3_byte_to_f32(src[i], 0);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = e4m3_byte_to_f32(src[i], 0);
 }
 __global__ void widen_kernel(uint16_t *__restrict__ dst, const uint8_t *__restrict__ src, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = f32_to_f16_bits(e4m3_to_f32(src[i]));
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = f32_to_f16_bits(e4m3_to_f32(src[i]));
 }
 
 int launch_quantize(uint8_t *dst, const void *src, int sd, int64_t n, hipStream_t st) {
     if (n == 0) return RM_OK;
-    quantize_kernel<<<dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st>>>(dst, src, sd, n);
+    quantize_kernel<<<dim3(stride_grid(n, 256)), dim3(256), 0, st>>>(dst, src, sd, n);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }
 int launch_dequantize(float *dst, const uint8_t *src, int64_t n, hipStream_t st) {
     if (n == 0) return RM_OK;
-    dequantize_kernel<<<dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st>>>(dst, src, n);
+    dequantize_kernel<<<dim3(stride_grid(n, 256)), dim3(256), 0, st>>>(dst, src, n);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }
 int launch_widen_f16(uint16_t *dst, const uint8_t *src, int64_t n, hipStream_t st) {
     if (n == 0) return RM_OK;
-    widen_kernel<<<dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st>>>(dst, src, n);
+    widen_kernel<<<dim3(stride_grid(n, 256)), dim3(256), 0, st>>>(dst, src, n);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }

@@ -542,7 +542,7 @@ static int launch_common(const GemmArgs &a0, const void *fn, size_t lds, int nt,
     a.items8 = cdiv(items, 8) * 8;
     int64_t G = a.items8;
     if (a.persist_grid >= 8) G = std::min<int64_t>(G, a.persist_grid / 8 * 8);
-    RM_CHECK(G < (int64_t)1 << 31, RM_ERR_SHAPE, "grid too large (%lld workgroups)", (long long)G);
+    RM_CHECK(G * nt <= kMaxWorkItems, RM_ERR_SHAPE, "grid too large (%lld workgroups)", (long long)G);
     RM_CHECK(lds <= 160 * 1024, RM_ERR_SHAPE, "fused kernel needs %zu B of LDS", lds);
     RM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     void *args[] = {(void *)&a};
@@ -720,6 +720,7 @@ int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int6
                  int64_t ldb, int K, float *vals, int32_t *locs, int64_t ldo, hipStream_t st) {
     if (M == 0) return RM_OK;
     RM_CHECK(S <= 256, RM_ERR_ARG, "at most 256 column splits (got %d)", S);
+    RM_CHECK(M * 64 <= kMaxWorkItems, RM_ERR_SHAPE, "%lld rows exceed one launch", (long long)M);
     size_t lds = (mode == RM_MODE_FP8_RESCORE) ? (size_t)K * 4 : 0;
     merge_kernel<<<dim3((unsigned)M), dim3(64), lds, st>>>(part_v, part_c, S, KP, M, N, ktop, mode,
                                                            q32, ldq32, db, ldb, K, vals, locs, ldo);
@@ -772,6 +773,8 @@ __global__ __launch_bounds__(64) void rowtopk_kernel(float *dv, int32_t *dl, int
 int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dtype, int64_t ldc,
                    int64_t M, int64_t wi, int64_t offset, int k, hipStream_t st) {
     if (M == 0) return RM_OK;
+    RM_CHECK(cdiv(M, 64) * 64 <= kMaxWorkItems, RM_ERR_SHAPE, "%lld rows exceed one launch",
+             (long long)M);
     size_t lds = (size_t)64 * 2 * k * 4;
     rowtopk_kernel<<<dim3((unsigned)cdiv(M, 64)), dim3(64), lds, st>>>(dv, dl, ldd, C, c_dtype, ldc,
                                                                        M, wi, offset, k);
@@ -782,19 +785,20 @@ int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dty
 // dst[r*cols + c] = src[r*s_row + c*s_col] for esize-byte elements.
 __global__ void transpose_kernel(uint8_t *dst, const uint8_t *src, int64_t rows, int64_t cols,
                                  int64_t s_row, int64_t s_col, int esize) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= rows * cols) return;
-    const int64_t r = g / cols, c = g % cols;
-    const uint8_t *s = src + (r * s_row + c * s_col) * esize;
-    uint8_t *d = dst + g * esize;
-    for (int b = 0; b < esize; ++b) d[b] = s[b];
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < rows * cols;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = g / cols, c = g % cols;
+        const uint8_t *s = src + (r * s_row + c * s_col) * esize;
+        uint8_t *d = dst + g * esize;
+        for (int b = 0; b < esize; ++b) d[b] = s[b];
+    }
 }
 
 int launch_transpose_u8(uint8_t *dst, const uint8_t *src, int64_t rows, int64_t cols, int64_t s_row,
                         int64_t s_col, int esize, hipStream_t st) {
     const int64_t n = rows * cols;
     if (n == 0) return RM_OK;
-    transpose_kernel<<<dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st>>>(dst, src, rows, cols, s_row,
+    transpose_kernel<<<dim3(stride_grid(n, 256)), dim3(256), 0, st>>>(dst, src, rows, cols, s_row,
                                                                          s_col, esize);
     RM_HIP(hipGetLastError());
     return RM_OK;

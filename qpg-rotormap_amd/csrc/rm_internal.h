@@ -31,6 +31,16 @@ void clear_error();
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// An HSA dispatch packet holds the grid size in work-items as 32 bits: a
+// launch must stay below 2^32 work-items.  Element-wise kernels therefore
+// walk their range with a grid stride over a grid of at most kStrideBlocks.
+constexpr int64_t kMaxWorkItems = 0xffffffffLL;
+constexpr int64_t kStrideBlocks = 1 << 20;
+inline unsigned stride_grid(int64_t n, int block) {
+    const int64_t g = cdiv(n, block);
+    return (unsigned)(g < kStrideBlocks ? g : kStrideBlocks);
+}
+
 // ---- fused GEMM + top-k tile geometry (shared by host planning and kernels)
 constexpr int kBI = 256;       // index columns per tile
 constexpr int kBK = 64;        // K bytes per LDS stage

@@ -101,8 +101,14 @@ def sample_reads(ref, n, k, err=0.0, seed=1, draw="even", nfree=True, model=0):
         ranks[:] = r - np.concatenate([[0], cum])[recs]
     else:
         raise L.RotorMapArgumentError(L.RM_ERR_ARG, "draw must be 'even' or 'random'")
-    starts0 = np.array([_rank_to_start(pools[j], int(q)) for j, q in zip(recs, ranks)],
-                       dtype=np.int64)
+    starts0 = np.empty(n, dtype=np.int64)
+    for j in np.unique(recs):
+        sel = np.where(recs == j)[0]
+        a_ = np.array([a for a, _ in pools[j]], dtype=np.int64)
+        ln = np.array([b - a for a, b in pools[j]], dtype=np.int64)
+        cum = np.cumsum(ln)
+        r = np.searchsorted(cum, ranks[sel], side="right")  # range holding each rank
+        starts0[sel] = a_[r] + ranks[sel] - (cum[r] - ln[r])
     W = (k + 15) // 16
     words = np.zeros((n, W), dtype=np.uint32)
     for j in np.unique(recs):

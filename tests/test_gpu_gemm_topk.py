@@ -226,3 +226,30 @@ def test_persistent_grid_bitexact(torch_gpu):
                 np.testing.assert_array_equal(v, ev, err_msg=f"vals, persistent={on}, S={S}")
     finally:
         L.check(R.lib().rm_set_persistent(ctx, 1))
+
+
+def test_index_past_2_32_elements(torch_gpu):
+    """2.1 M columns x 2048 = 4.3 G elements, past the 32-bit work-item count
+    of one dispatch: the FP16 copy is widened over the whole range and the
+    last columns are found by both paths (a random column's self product
+    beats every cross product by ~10x)."""
+    torch = torch_gpu
+    N, K = 2_100_000, 2048
+    B = torch.empty(N, K, dtype=torch.uint8, device="cuda")
+    for r0 in range(0, N, 1 << 18):  # random signed e4m3 codes of magnitude < 0x30
+        x = torch.randint(0, 96, (min(1 << 18, N - r0), K), dtype=torch.uint8, device="cuda",
+                          generator=torch.Generator(device="cuda").manual_seed(r0))
+        B[r0:r0 + x.shape[0]] = (x % 48) | ((x >= 48).to(torch.uint8) << 7)
+    rows = torch.tensor([0, 1, 2_097_151, 2_097_152, 2_099_998, 2_099_999], device="cuda")
+    want = rows.cpu().numpy() + 1
+    Q8 = B[rows].contiguous()
+    Q32 = torch.empty(len(rows), K, dtype=torch.float32, device="cuda")
+    R.f8_to_f32(Q32, Q8)
+    torch.cuda.synchronize()
+    for mode, q in (("fp8", Q8), ("fp16", Q32.half())):
+        eng = R.TopKEngine(R.ResidentIndex(B, fp16=(mode == "fp16")), k=4, mode=mode)
+        v, l = eng.batch_gemm_topk(q)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(l.cpu().numpy()[:, 0], want, err_msg=mode)
+        del eng
+        torch.cuda.empty_cache()
