@@ -19,13 +19,15 @@ Workloads (synthetic; one step = one batch of reads resident in HBM):
                     [0, 0.10]) one launch per batch vs one launch in all, after
                     the autotuner
   step              encode -> fused e4m3 GEMM + top-32 -> FP32 rescore -> top-20
+  fasta             the batch written as FASTA text (80-column lines), parsed and
+                    2-bit packed on the GPU; checked equal to the packed reads
   serving           Poisson request traces at 50 / 200 / 1000 req/s (human: also
                     10 req/s, below saturation; reads per request log-uniform
                     on [1, 100000]) through the continuous-batching server:
                     p50 / p99 / p999 latency, offered vs achieved reads/s
 
 python bench.py [--workload human|100mbp] [--steps K] [--warmup W] [--gpus 1]
-                [--reads M] [--no-cpu] [--no-serve] [--no-uneven]
+                [--reads M] [--no-cpu] [--no-serve] [--no-uneven] [--no-fasta]
 """
 import argparse
 import json
@@ -76,6 +78,31 @@ def traffic_from_profile(name):
         return None
 
 
+def fasta_text(words, k, width=80):
+    """Packed reads -> FASTA text: '>read_%08d' headers, sequence lines of
+    `width` bases (built in chunks of 8192 reads)."""
+    n = words.shape[0]
+    nl = -(-k // width)
+    rows = []
+    shifts = 2 * np.arange(16, dtype=np.uint32)
+    for c0 in range(0, n, 8192):
+        w = words[c0:c0 + 8192]
+        m = w.shape[0]
+        codes = ((w[:, :, None] >> shifts) & 3).astype(np.uint8).reshape(m, -1)[:, :k]
+        pad = np.full((m, nl * width), ord("\n"), np.uint8)
+        pad[:, :k] = np.frombuffer(b"ACGT", np.uint8)[codes]
+        body = np.concatenate([pad.reshape(m, nl, width), np.full((m, nl, 1), ord("\n"), np.uint8)],
+                              axis=2).reshape(m, -1)[:, :k + nl]
+        idx = np.arange(c0, c0 + m)
+        head = np.zeros((m, 15), np.uint8)
+        head[:, :6] = np.frombuffer(b">read_", np.uint8)
+        for d in range(8):
+            head[:, 6 + d] = ord("0") + (idx // 10 ** (7 - d)) % 10
+        head[:, 14] = ord("\n")
+        rows.append(np.concatenate([head, body], axis=1).reshape(-1))
+    return np.concatenate(rows)
+
+
 def rnd(x, n=3):
     return round(float(x), n)
 
@@ -92,6 +119,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-fasta", action="store_true")
     ap.add_argument("--serve-rates", default=None,
                     help="Poisson arrival rates, req/s (100mbp: 50,200,1000; human: 10,50,200,1000)")
     ap.add_argument("--serve-requests", default=None,
@@ -247,6 +275,43 @@ def main():
     }
     del eng16
     torch.cuda.empty_cache()
+
+    # ---- FASTA ingestion: the batch as FASTA text, parsed and packed on the GPU
+    if not args.no_fasta:
+        W = (K_FRAG + 15) // 16
+        nfa = M
+        width = 80
+        text = fasta_text(pool.words[:nfa], K_FRAG, width)
+        pin = torch.from_numpy(text).pin_memory()
+        draw = torch.empty(text.size, dtype=torch.uint8, device=dev)
+        torch.cuda.synchronize()
+        th = time.perf_counter()
+        draw.copy_(pin, non_blocking=True)
+        torch.cuda.synchronize()
+        t_h2d = time.perf_counter() - th
+        fw = torch.empty(nfa, W, dtype=torch.int32, device=dev)
+        nread = L.C.c_int64()
+        st = L.stream_of(0)
+        L.check(L.lib().rm_fasta_reads(ctx, L.ptr(draw), text.size, K_FRAG, L.ptr(fw), nfa, None,
+                                       None, L.C.byref(nread), st))  # warm-up
+        torch.cuda.synchronize()
+        reps = 5
+        tp = time.perf_counter()
+        for _ in range(reps):
+            L.check(L.lib().rm_fasta_reads(ctx, L.ptr(draw), text.size, K_FRAG, L.ptr(fw), nfa, None,
+                                           None, L.C.byref(nread), st))
+        torch.cuda.synchronize()
+        t_parse = (time.perf_counter() - tp) / reps
+        same = bool(nread.value == nfa and torch.equal(fw, words_all[:nfa]))
+        result["fasta_ingest"] = {
+            "bytes": int(text.size), "reads": int(nread.value), "line_width": width,
+            "gpu_parse_s": rnd(t_parse, 5), "gpu_parse_gb_s": rnd(text.size / t_parse / 1e9, 1),
+            "h2d_gb_s": rnd(text.size / t_h2d / 1e9, 1),
+            "words_equal_packed_reads": same}
+        log(f"fasta ingest: {text.size / 1e9:.2f} GB parsed in {t_parse * 1e3:.2f} ms "
+            f"({text.size / t_parse / 1e9:.0f} GB/s on the GPU, H2D {text.size / t_h2d / 1e9:.1f} GB/s); "
+            f"round trip {'ok' if same else 'MISMATCH'}")
+        del pin, draw, fw, text
 
     # ---- uneven batches: one launch per batch vs one launch for all ---------
     if human and not args.no_uneven:

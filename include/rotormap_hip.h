@@ -248,6 +248,22 @@ int rm_synth_record(uint64_t seed, int64_t rec, int64_t len, double gc, uint32_t
 /* Set bases [lo, hi) of a packed record to code 2: N runs as the reader maps
  * them (RotorMap/fasta/reader.jl:21-28, N -> G). */
 int rm_mark_n(uint32_t *words, int64_t lo, int64_t hi);
+/* FASTA bytes (device, n_bytes) -> packed reads on the GPU, the parse of
+ * fasta_reads (RotorMap/fasta/reader.jl:321-350 with _LUT3 :21-28, record
+ * split _next_rec3 :61-76, header / trim / skip _part_task3 :204-300, packing
+ * _emit3! :163-197): records start at a '>' at a line start; bytes before
+ * the first record are ignored; the header is '>' .. the byte before the
+ * first '\n' (one trailing '\r' dropped); every other byte but '\n' / '\r'
+ * is a base (A/a 0, C/c 1, G/g 2, T/t 3, anything else 2); records with >= k
+ * bases are trimmed to k and kept, in file order, shorter ones skipped.
+ * words (device): max_reads x ceil(k/16) uint32 rows; hdr_off / hdr_len
+ * (device, may be NULL): each kept read's header as a byte range of raw.
+ * *n_reads (host) = reads kept; RM_ERR_SHAPE (with *n_reads set) when more
+ * than max_reads are kept.  Synchronises `stream`.  raw may be split between
+ * calls only at a line-start '>'. */
+int rm_fasta_reads(rm_ctx *ctx, const uint8_t *raw, int64_t n_bytes, int k, uint32_t *words,
+                   int64_t max_reads, int64_t *hdr_off, int32_t *hdr_len, int64_t *n_reads,
+                   void *stream);
 /* Host twin of rm_extract_windows (host pointers). */
 int rm_extract_windows_host(uint32_t *out, const uint32_t *rec_words, const int64_t *starts,
                             int64_t n, int k);

@@ -47,6 +47,7 @@ def lib():
             "rmo_topk_rows_f32": (None, [vp, i64, i64, i64, i32, vp, vp]),
             "rmo_rescore_dot": (C.c_float, [vp, vp, i32]),
             "rmo_rescore_rank": (None, [vp, vp, i32, vp, i32, i32, vp, vp]),
+            "rmo_fasta_reads": (i64, [vp, i64, i32, vp, i64, vp, vp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(o, n)
@@ -193,3 +194,18 @@ def rescore_dot(q32, col):
     q = np.ascontiguousarray(q32, dtype=np.float32)
     b = np.ascontiguousarray(col, dtype=np.uint8)
     return lib().rmo_rescore_dot(_p(q), _p(b), len(q))
+
+
+def fasta_reads(raw, k):
+    """(words (n, ceil(k/16)) uint32, headers list[bytes]) of a FASTA buffer."""
+    raw = np.ascontiguousarray(np.frombuffer(raw, np.uint8) if isinstance(raw, (bytes, bytearray))
+                               else raw, dtype=np.uint8)
+    cap = int(np.count_nonzero(raw == ord(">"))) + 1
+    W = (k + 15) // 16
+    words = np.zeros((cap, W), np.uint32)
+    hoff = np.zeros(cap, np.int64)
+    hlen = np.zeros(cap, np.int32)
+    n = lib().rmo_fasta_reads(_p(raw) if raw.size else None, raw.size, k, _p(words), cap,
+                              _p(hoff), _p(hlen))
+    heads = [raw[o:o + l].tobytes() for o, l in zip(hoff[:n], hlen[:n])]
+    return words[:n], heads

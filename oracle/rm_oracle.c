@@ -62,6 +62,57 @@ uint8_t rmo_byte_code(uint8_t b) {
     }
 }
 
+/* FASTA parse of one byte buffer, restating fasta_reads on a single part
+ * (RotorMap/fasta/reader.jl): record starts = '>' at a line start (byte 0 or
+ * after '\n' / '\r', _next_rec3 :61-76); bytes before the first record are
+ * ignored (:247-250); header = '>' .. the byte before the first '\n', one
+ * trailing '\r' dropped (:256-261); sequence bytes = the rest of the record
+ * minus '\n' / '\r', mapped by _LUT3 (:21-28, _collect3! :105-160); a record
+ * with >= k bases is trimmed to k and emitted (:266-270, _emit3! :163-197).
+ * A header line with no '\n' before the next record (a bare '\r' line end)
+ * yields no read and the parse goes on with the next record (the reference
+ * ends its byte range there, so what it drops depends on the thread split).
+ * Writes up to max_reads reads (words: W = ceil(k/16) per read) and their
+ * header byte ranges; returns the number of reads kept. */
+static int rmo_is_eol(uint8_t b) { return b == '\n' || b == '\r'; }
+
+static int64_t rmo_next_rec(const uint8_t *raw, int64_t from, int64_t n) {
+    for (int64_t p = from; p < n; ++p)
+        if (raw[p] == '>' && (p == 0 || rmo_is_eol(raw[p - 1]))) return p;
+    return n;
+}
+
+int64_t rmo_fasta_reads(const uint8_t *raw, int64_t n, int k, uint32_t *words, int64_t max_reads,
+                        int64_t *hoff, int32_t *hlen) {
+    const int W = (k + 15) / 16;
+    uint8_t *codes = (uint8_t *)malloc((size_t)k);
+    int64_t cnt = 0;
+    int64_t s = rmo_next_rec(raw, 0, n);
+    while (s < n) {
+        const int64_t e = rmo_next_rec(raw, s + 1, n);
+        int64_t h = s;
+        while (h < e && raw[h] != '\n') ++h;
+        if (h < e) {
+            int64_t hend = h - 1;
+            if (hend >= s && raw[hend] == '\r') --hend;
+            int b = 0;
+            for (int64_t i = h + 1; i < e && b < k; ++i)
+                if (!rmo_is_eol(raw[i])) codes[b++] = rmo_byte_code(raw[i]);
+            if (b >= k) {
+                if (cnt < max_reads) {
+                    rmo_forward_pack(codes, k, words + cnt * W);
+                    hoff[cnt] = s;
+                    hlen[cnt] = (int32_t)(hend - s + 1);
+                }
+                ++cnt;
+            }
+        }
+        s = e;
+    }
+    free(codes);
+    return cnt;
+}
+
 /* Reverse complement of a k-base fragment, codes -> codes.
  * RotorMap/index/build.jl:415-422 (_revcomp_words_ref): rc[j] = 3 - codes[k-1-j]. */
 void rmo_revcomp_codes(const uint8_t *codes, int k, uint8_t *rc) {

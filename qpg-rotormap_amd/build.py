@@ -27,6 +27,7 @@ HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off"]
 SOURCES = [
     ("encode.hip", "hip"),
     ("gemm_topk.hip", "hip"),
+    ("fasta.hip", "hip"),
     ("capi.cpp", "hipcc-host"),
     ("serve.cpp", "hipcc-host"),
     ("synth.cpp", "host"),

@@ -288,6 +288,20 @@ int rm_extract_windows(rm_ctx *ctx, uint32_t *out, const uint32_t *rec_words,
     return launch_extract(out, rec_words, starts, n, k, pick_stream(ctx, stream));
 }
 
+int rm_fasta_reads(rm_ctx *ctx, const uint8_t *raw, int64_t n_bytes, int k, uint32_t *words,
+                   int64_t max_reads, int64_t *hdr_off, int32_t *hdr_len, int64_t *n_reads,
+                   void *stream) {
+    RM_CHECK(ctx && n_reads, RM_ERR_ARG, "NULL argument");
+    *n_reads = 0;
+    RM_CHECK(k >= 1 && k <= 262144, RM_ERR_ARG, "k must be in 1..262144 (got %d)", k);
+    RM_CHECK(n_bytes >= 0 && max_reads >= 0, RM_ERR_ARG, "negative size");
+    RM_CHECK(n_bytes == 0 || raw, RM_ERR_ARG, "raw is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    RM_HIP(hipSetDevice(ctx->device));
+    return fasta_reads(ctx, raw, n_bytes, k, words, max_reads, hdr_off, hdr_len, n_reads,
+                       pick_stream(ctx, stream));
+}
+
 int rm_quantize_e4m3(rm_ctx *ctx, uint8_t *dst, const void *src, int src_dtype, int64_t n,
                      void *stream) {
     RM_CHECK(ctx && dst && src, RM_ERR_ARG, "NULL argument");

@@ -10,6 +10,7 @@ from ._lib import (RM_E4M3, RM_F16, RM_F32, RM_MODE_FP8, RM_MODE_FP8_RESCORE, RM
 from .encode import (RopeEncoder, dim, encode_frag_real_batch, encode_frag_real_batch_alloc,
                      phase_table, rdim, revcomp_frag_words)
 from .gemm import f8_to_f32, f32_to_f8, fp8_gemm_hip, launch_seg_rowtopk_merge
+from .fasta import fasta_reads_gpu, iter_fasta_gpu
 from .genome import Reference, Record, pack_codes, read_fasta, synth_reference, unpack_words, write_fasta
 from .index import IndexData, build_index, load_index, save_index, window_table
 from .reads import ReadSet, even_alloc, parse_read_head, sample_reads
