@@ -264,6 +264,12 @@ int rm_mark_n(uint32_t *words, int64_t lo, int64_t hi);
 int rm_fasta_reads(rm_ctx *ctx, const uint8_t *raw, int64_t n_bytes, int k, uint32_t *words,
                    int64_t max_reads, int64_t *hdr_off, int32_t *hdr_len, int64_t *n_reads,
                    void *stream);
+/* rm_fasta_reads on host buffers (raw, words, hdr_off, hdr_len all host; the
+ * library stages them through its workspace): for callers without device
+ * arrays, such as a Julia process without a GPU array package. */
+int rm_fasta_reads_host(rm_ctx *ctx, const uint8_t *raw, int64_t n_bytes, int k,
+                        uint32_t *words, int64_t max_reads, int64_t *hdr_off, int32_t *hdr_len,
+                        int64_t *n_reads);
 /* Host twin of rm_extract_windows (host pointers). */
 int rm_extract_windows_host(uint32_t *out, const uint32_t *rec_words, const int64_t *starts,
                             int64_t n, int k);

@@ -302,6 +302,38 @@ int rm_fasta_reads(rm_ctx *ctx, const uint8_t *raw, int64_t n_bytes, int k, uint
                        pick_stream(ctx, stream));
 }
 
+int rm_fasta_reads_host(rm_ctx *ctx, const uint8_t *raw, int64_t n_bytes, int k,
+                        uint32_t *words, int64_t max_reads, int64_t *hdr_off, int32_t *hdr_len,
+                        int64_t *n_reads) {
+    RM_CHECK(ctx && n_reads, RM_ERR_ARG, "NULL argument");
+    *n_reads = 0;
+    RM_CHECK(k >= 1 && k <= 262144, RM_ERR_ARG, "k must be in 1..262144 (got %d)", k);
+    RM_CHECK(n_bytes >= 0 && max_reads >= 0, RM_ERR_ARG, "negative size");
+    RM_CHECK(n_bytes == 0 || raw, RM_ERR_ARG, "raw is NULL");
+    if (n_bytes == 0) return RM_OK;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    RM_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int64_t W = (k + 15) / 16;
+    void *draw, *dw, *doff, *dlen;
+    int rc;
+    if ((rc = ws_get(ctx, "fah_raw", (size_t)n_bytes, &draw))) return rc;
+    if ((rc = ws_get(ctx, "fah_words", (size_t)std::max<int64_t>(1, max_reads) * W * 4, &dw))) return rc;
+    if ((rc = ws_get(ctx, "fah_off", (size_t)std::max<int64_t>(1, max_reads) * 8, &doff))) return rc;
+    if ((rc = ws_get(ctx, "fah_len", (size_t)std::max<int64_t>(1, max_reads) * 4, &dlen))) return rc;
+    RM_HIP(hipMemcpyAsync(draw, raw, (size_t)n_bytes, hipMemcpyHostToDevice, st));
+    rc = fasta_reads(ctx, (const uint8_t *)draw, n_bytes, k, (uint32_t *)dw, max_reads,
+                     (int64_t *)doff, (int32_t *)dlen, n_reads, st);
+    if (rc) return rc;
+    const int64_t n = *n_reads;
+    if (n > 0 && words)
+        RM_HIP(hipMemcpyAsync(words, dw, (size_t)n * W * 4, hipMemcpyDeviceToHost, st));
+    if (n > 0 && hdr_off) RM_HIP(hipMemcpyAsync(hdr_off, doff, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    if (n > 0 && hdr_len) RM_HIP(hipMemcpyAsync(hdr_len, dlen, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    RM_HIP(hipStreamSynchronize(st));
+    return RM_OK;
+}
+
 int rm_quantize_e4m3(rm_ctx *ctx, uint8_t *dst, const void *src, int src_dtype, int64_t n,
                      void *stream) {
     RM_CHECK(ctx && dst && src, RM_ERR_ARG, "NULL argument");

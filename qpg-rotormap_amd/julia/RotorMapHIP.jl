@@ -16,7 +16,7 @@
 module RotorMapHIP
 
 export HipContext, HipIndex, HipServer, rope_topk_stream_hip, map_words_hip!,
-       hip_encode_batch!, hip_version
+       hip_encode_batch!, hip_version, fasta_reads_hip
 
 const LIB = get(ENV, "ROTORMAP_HIP_LIB",
                 joinpath(@__DIR__, "..", "rotormap_amd", "librotormap_hip.so"))
@@ -206,6 +206,70 @@ function rope_topk_stream_hip(re, file::String, Bh::AbstractMatrix;
     finally
         isopen(out) && close(out)
         finalize(srv)
+    end
+    bind(out, task)
+    return out
+end
+
+"""
+fasta_reads_hip(file; k, chunk_bytes, device) -> Channel
+
+The GPU parse of `file` (rm_fasta_reads_host): the same fragments as the
+reference's fasta_reads (RotorMap/fasta/reader.jl:321-350) -- named tuples
+(header, len = k, words = cld(k, 16) UInt32), kept records in file order.
+The file is read in chunks cut at a line-start '>'.  Usable as the `reader`
+of rope_topk_stream_hip.
+"""
+function fasta_reads_hip(file::AbstractString; k::Integer = 20_000, parts::Integer = 1,
+                         chunk_bytes::Integer = 1 << 30, device::Integer = 0)
+    ctx = HipContext(device)
+    W = cld(Int(k), 16)
+    out = Channel{Any}(64)
+    task = Threads.@spawn try
+        open(file, "r") do io
+            carry = UInt8[]
+            while true
+                chunk = read(io, chunk_bytes)
+                eof_ = isempty(chunk)
+                buf = isempty(carry) ? chunk : vcat(carry, chunk)
+                cut = length(buf)
+                if !eof_
+                    cut = 0
+                    for p in length(buf):-1:2  # last line-start '>'
+                        if buf[p] == UInt8('>') && (buf[p - 1] == UInt8('\n') || buf[p - 1] == UInt8('\r'))
+                            cut = p - 1
+                            break
+                        end
+                    end
+                    if cut == 0
+                        carry = buf
+                        continue
+                    end
+                end
+                if cut > 0
+                    cap = count(==(UInt8('>')), view(buf, 1:cut)) + 1
+                    words = Matrix{UInt32}(undef, W, cap)
+                    off = Vector{Int64}(undef, cap)
+                    len = Vector{Int32}(undef, cap)
+                    n = Ref{Int64}(0)
+                    GC.@preserve buf check(ccall((:rm_fasta_reads_host, LIB), Cint,
+                        (Ptr{Cvoid}, Ptr{UInt8}, Int64, Cint, Ptr{UInt32}, Int64, Ptr{Int64},
+                         Ptr{Int32}, Ref{Int64}),
+                        ctx.ptr, pointer(buf), cut, k, words, cap, off, len, n))
+                    for i in 1:n[]
+                        hdr = String(buf[off[i] + 1:off[i] + len[i]])
+                        put!(out, (header = hdr, len = Int(k), words = words[:, i]))
+                    end
+                end
+                carry = buf[cut + 1:end]
+                eof_ && break
+            end
+        end
+    catch e
+        close(out, e)
+        rethrow()
+    finally
+        isopen(out) && close(out)
     end
     bind(out, task)
     return out

@@ -77,6 +77,7 @@ SIGNATURES = {
     "rm_server_stats": (i32, [vp, vp, vp, vp]),
     "rm_server_destroy": (i32, [vp]),
     "rm_fasta_reads": (i32, [vp, vp, i64, i32, vp, i64, vp, vp, C.POINTER(i64), vp]),
+    "rm_fasta_reads_host": (i32, [vp, vp, i64, i32, vp, i64, vp, vp, C.POINTER(i64)]),
     "rm_synth_record": (i32, [u64, i64, i64, C.c_double, vp]),
     "rm_mark_n": (i32, [vp, i64, i64]),
     "rm_extract_windows_host": (i32, [vp, vp, vp, i64, i32]),

@@ -155,3 +155,83 @@ def load_index(path):
     assert e8.shape == (2 * n_frag, rd), "embeds8 shape mismatch"
     return IndexData(e8, z["norms"], heads, z["starts"], z["strand"], k, kstep, s, m, c, normalize,
                      n_frag, str(z["source"]), FP8_FORMAT, rec_of_col, names)
+
+
+def _names_and_recs(heads):
+    names, idx, rec = [], {}, np.empty(len(heads), dtype=np.int64)
+    for i, h in enumerate(heads):
+        nm = h.split()[0][1:] if h.split() else ""
+        if nm not in idx:
+            idx[nm] = len(names)
+            names.append(nm)
+        rec[i] = idx[nm]
+    return names, rec
+
+
+def save_index_dir(d, path):
+    """Write d in the raw-array layout of julia/export_index.jl (meta.json,
+    embeds8.u8, norms.f32, starts.i64, strand.u8, heads.txt)."""
+    import json
+    import os
+    os.makedirs(path, exist_ok=True)
+    N, rd = d.embeds8.shape
+    norms = np.ascontiguousarray(d.norms, dtype=np.float32)  # (N, m) = the (m, N) column-major
+    meta = dict(format=d.format, source=d.source, k=d.k, kstep=d.kstep, s=d.s, m=d.m, c=d.c,
+                normalize=d.normalize, n_frag=d.n_frag, rdim=rd, N=N,
+                norms_shape=[norms.shape[1], norms.shape[0]])
+    with open(os.path.join(path, "meta.json"), "w") as f:
+        json.dump(meta, f)
+    np.ascontiguousarray(d.embeds8, dtype=np.uint8).tofile(os.path.join(path, "embeds8.u8"))
+    norms.tofile(os.path.join(path, "norms.f32"))
+    np.asarray(d.starts, dtype=np.int64).tofile(os.path.join(path, "starts.i64"))
+    np.asarray(d.strand, dtype=np.uint8).tofile(os.path.join(path, "strand.u8"))
+    with open(os.path.join(path, "heads.txt"), "w", encoding="latin-1") as f:
+        for h in d.heads:
+            f.write(h + "\n")
+
+
+def load_index_dir(path, device=None):
+    """Read an index exported by julia/export_index.jl (or save_index_dir).
+    An "indexreal.v1" export (fp16 rows) is quantised to the e4m3 database
+    on the GPU, e4m3(fp32(fp16)) as index_database_host does (RotorMap/index/
+    load.jl:130-143); normalize must be 0 (RotorMap/index/load.jl:57-59)."""
+    import json
+    import os
+    with open(os.path.join(path, "meta.json")) as f:
+        meta = json.load(f)
+    fmt, N, rd = meta["format"], int(meta["N"]), int(meta["rdim"])
+    if fmt not in (FP8_FORMAT, "indexreal.v1"):
+        raise L.RotorMapArgumentError(L.RM_ERR_ARG, f"not an indexreal index: {fmt}")
+    if meta["normalize"] != 0:
+        raise L.RotorMapArgumentError(L.RM_ERR_ARG, "the mapping flow assumes a normalize = 0 index")
+    if rd != 2 * meta["m"] * 4 ** meta["c"]:
+        raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "rdim does not match 2*m*4^c")
+    if fmt == FP8_FORMAT:
+        e8 = np.fromfile(os.path.join(path, "embeds8.u8"), dtype=np.uint8)
+        if e8.size != N * rd:
+            raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "embeds8.u8 size mismatch")
+        e8 = e8.reshape(N, rd)
+    else:
+        import torch
+        e16 = np.fromfile(os.path.join(path, "embeds16.f16"), dtype=np.float16)
+        if e16.size != N * rd:
+            raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "embeds16.f16 size mismatch")
+        from .gemm import f32_to_f8
+        dev = torch.device(f"cuda:{device or 0}")
+        src = torch.from_numpy(e16).to(dev)
+        dst = torch.empty(N * rd, dtype=torch.uint8, device=dev)
+        f32_to_f8(dst, src)
+        e8 = dst.cpu().numpy().reshape(N, rd)
+    m_, n_ = meta.get("norms_shape", [meta["m"], N])
+    norms = np.fromfile(os.path.join(path, "norms.f32"), dtype=np.float32)
+    norms = norms.reshape(n_, m_) if m_ == meta["m"] else norms.reshape(m_, n_).T
+    starts = np.fromfile(os.path.join(path, "starts.i64"), dtype=np.int64)
+    strand = np.fromfile(os.path.join(path, "strand.u8"), dtype=np.uint8)
+    with open(os.path.join(path, "heads.txt"), encoding="latin-1") as f:
+        heads = f.read().split("\n")[:N]
+    if not (len(heads) == len(starts) == len(strand) == N):
+        raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "meta length mismatch")
+    names, rec = _names_and_recs(heads)
+    return IndexData(e8, norms, heads, starts, strand, int(meta["k"]), int(meta["kstep"]),
+                     int(meta["s"]), int(meta["m"]), int(meta["c"]), 0, int(meta["n_frag"]),
+                     meta.get("source", ""), FP8_FORMAT, rec, names)

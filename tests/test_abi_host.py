@@ -163,3 +163,29 @@ def test_extract_windows_host_matches_unpack():
     codes = R.unpack_words(rec.words[:-1], rec.length)
     for i, s in enumerate(starts):
         np.testing.assert_array_equal(R.unpack_words(out[i], k), codes[s:s + k])
+
+
+def test_index_dir_round_trip(tmp_path):
+    """save_index_dir / load_index_dir (the julia/export_index.jl layout)."""
+    rng = np.random.default_rng(3)
+    N, rd, m = 40, 2048, 1
+    heads = [f">chr{1 + (i % 3)} synthetic record" for i in range(N)]
+    d = R.IndexData(rng.integers(0, 256, (N, rd), dtype=np.uint8), rng.random((N, m), dtype=np.float32),
+                    heads, np.arange(1, N + 1, dtype=np.int64) * 7, (np.arange(N) % 2).astype(np.uint8),
+                    20000, 2000, 5, m, 5, 0, N // 2, "/data/ref.fa", "indexreal.fp8.v1")
+    p = str(tmp_path / "idx.rmidx")
+    R.save_index_dir(d, p)
+    e = R.load_index_dir(p)
+    np.testing.assert_array_equal(e.embeds8, d.embeds8)
+    np.testing.assert_array_equal(e.norms, d.norms)
+    np.testing.assert_array_equal(e.starts, d.starts)
+    np.testing.assert_array_equal(e.strand, d.strand)
+    assert e.heads == heads and e.rec_names == ["chr1", "chr2", "chr3"]
+    assert e.rec_of_col.tolist() == [i % 3 for i in range(N)]
+    assert (e.k, e.kstep, e.s, e.m, e.c, e.n_frag, e.source) == (20000, 2000, 5, 1, 5, 20, "/data/ref.fa")
+    import json
+    meta = json.load(open(tmp_path / "idx.rmidx" / "meta.json"))
+    meta["normalize"] = 1
+    json.dump(meta, open(tmp_path / "idx.rmidx" / "meta.json", "w"))
+    with pytest.raises(R.RotorMapArgumentError):
+        R.load_index_dir(p)

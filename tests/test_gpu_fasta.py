@@ -105,3 +105,19 @@ def test_iter_fasta_gpu_chunks(torch_gpu, tmp_path):
     assert got_h == oh
     np.testing.assert_array_equal(np.concatenate(got_w), ow)
     assert firsts[0] == 1 and firsts[1] == 38
+
+
+def test_fasta_host_entry(torch_gpu):
+    data = make_fasta(8, 60, 2500)
+    raw = np.frombuffer(data, np.uint8).copy()
+    cap, W = data.count(b">") + 1, (2500 + 15) // 16
+    words = np.zeros((cap, W), np.uint32)
+    off = np.zeros(cap, np.int64)
+    ln = np.zeros(cap, np.int32)
+    n = C.c_int64()
+    L.check(L.lib().rm_fasta_reads_host(L.ctx(0), L.ptr(raw), raw.size, 2500, L.ptr(words), cap,
+                                        L.ptr(off), L.ptr(ln), C.byref(n)))
+    ow, oh = O.fasta_reads(data, 2500)
+    assert n.value == len(oh)
+    np.testing.assert_array_equal(words[:n.value], ow)
+    assert [raw[o:o + l].tobytes() for o, l in zip(off[:n.value], ln[:n.value])] == oh
