@@ -380,6 +380,17 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
         const int lim = (int)min<int64_t>(a.N - i0 - rowbase, (int64_t)kBI);
         const int32_t colbase = (int32_t)(a.col_base + i0 + rowbase);
         const bool full = i0 + kBI <= a.N;  // uniform: all columns in range
+        if (!full) {
+            // columns past N (rows read as zeros) score -inf: never above tau,
+            // and they can only lower the first-tile bound
+#pragma unroll
+            for (int ai = 0; ai < 4; ++ai)
+#pragma unroll
+                for (int bj = 0; bj < BB; ++bj)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i)
+                        if (RC(ai, i) >= lim) acc[ai][bj][i] = -INFINITY;
+        }
 
         // --- first tile of the item: seed tau with a lower bound of each
         // query's k'-th value, so the first tiles do not append every value.
@@ -398,7 +409,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                     float m1 = -INFINITY, m2 = -INFINITY;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
-                        const float v = RC(ai, i) < lim ? acc[ai][bj][i] : -INFINITY;
+                        const float v = acc[ai][bj][i];
                         m2 = fmaxf(m2, fminf(m1, v));
                         m1 = fmaxf(m1, v);
                     }
@@ -431,25 +442,17 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
             pend[bj] = 0;
             const float tau = tauv[ql[bj]];
             float mx = -INFINITY;
-            if (full) {  // every column of the tile is in range: plain max
 #pragma unroll
-                for (int ai = 0; ai < 4; ++ai)
+            for (int ai = 0; ai < 4; ++ai)
 #pragma unroll
-                    for (int i = 0; i < 16; i += 2)
-                        mx = fmaxf(mx, fmaxf(acc[ai][bj][i], acc[ai][bj][i + 1]));
-            } else {
-#pragma unroll
-                for (int ai = 0; ai < 4; ++ai)
-#pragma unroll
-                    for (int i = 0; i < 16; ++i)
-                        if (RC(ai, i) < lim) mx = fmaxf(mx, acc[ai][bj][i]);
-            }
+                for (int i = 0; i < 16; i += 2)
+                    mx = fmaxf(mx, fmaxf(acc[ai][bj][i], acc[ai][bj][i + 1]));
             if (qv[bj] && mx > tau) {
 #pragma unroll
                 for (int ai = 0; ai < 4; ++ai)
 #pragma unroll
                     for (int i = 0; i < 16; ++i)
-                        if (RC(ai, i) < lim && acc[ai][bj][i] > tau) pend[bj] |= 1ull << (ai * 16 + i);
+                        if (acc[ai][bj][i] > tau) pend[bj] |= 1ull << (ai * 16 + i);
             }
         }
         uint64_t any = 0;
