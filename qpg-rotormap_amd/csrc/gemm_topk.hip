@@ -15,20 +15,25 @@
 // ONE query (lane % 32) and 16 index columns -- the per-query top-k needs no
 // cross-lane traffic to filter.
 //
-// Tile: 256 index rows x 256 queries per workgroup (4 waves in 2x2, wave
-// tile 128 x 128 = 4 x 4 blocks of v_mfma_f32_32x32x64_f8f6f4, or 2 x
-// v_mfma_f32_32x32x16_f16 per 64-byte stage for the FP16 path), K streamed
-// through double-buffered LDS in 64-byte stages (80-byte row pitch: the
-// ds_read_b128 fragment reads of 8 consecutive rows hit 8 distinct 4-bank
-// groups).  A workgroup owns one query tile and walks a contiguous range of
-// index tiles (its column split); per query it keeps a top-k' list in LDS
-// and a threshold tau = the list's k'-th value.  After each tile a lane
-// flags values > tau; only flagged values (rare once tau has risen) are
-// appended to per-query LDS candidate slots and merged by one thread per
-// query under the (score desc, column asc) order.  Workgroups are mapped so
-// that the 32 concurrently resident workgroups of one XCD (dispatch is
-// round-robin over the 8 XCDs) share a column split: each index tile is
-// fetched into that XCD's L2 once and reused by all its query tiles.
+// Tile: 256 index rows x 256 queries per workgroup, 8 waves (two per SIMD)
+// in 2 x 4, wave tile 128 index rows x 64 queries = 4 x 2 blocks of
+// v_mfma_scale_f32_32x32x64_f8f6f4 (unit scales) per 64-byte K stage, or
+// 2 x v_mfma_f32_32x32x16_f16 per block for the FP16 path; lists longer than
+// 32 use 4 waves over 256 x 128 tiles.  K is streamed through double-buffered
+// LDS in 64-byte stages (64-byte rows, 16-byte chunks XOR-swizzled by
+// (row >> 2) & 3, conflict-free for the fragment reads), global -> register
+// loads two stages ahead, register -> LDS stores one stage ahead, one barrier
+// per stage.  A workgroup owns one query tile and walks a contiguous range
+// of index tiles (its column split); per query it keeps a top-k' list in LDS
+// and a threshold tau = the list's k'-th value.  After each tile a lane flags
+// values > tau (a query's 16 x 4 values are compared one by one only when
+// their maximum beats tau); flagged values are appended to per-query LDS
+// slots that persist across tiles and are merged into the lists (one thread
+// per query, (score desc, column asc) order) only when some query's slots
+// overflow.  The grid is persistent by default (one workgroup per CU walking
+// its XCD's items); items are ordered so that the concurrently resident
+// workgroups of one XCD (dispatch is round-robin over the 8 XCDs) share
+// index tiles through that XCD's L2.
 #include "rm_device.h"
 #include "rm_internal.h"
 
@@ -156,8 +161,10 @@ struct Bytes32 {
     uint4 lo, hi;
 };
 
-template <int ELEM, int BB, int A0, int A1>
+// FIRST: the tile's first stage, accumulating onto zero (no zeroing pass)
+template <int ELEM, int BB, int A0, int A1, bool FIRST = false>
 __device__ __forceinline__ void mma_part(v16f (&acc)[4][BB], const Frags<BB> &f) {
+    const v16f zero = {};
 #pragma unroll
     for (int ai = A0; ai < A1; ++ai)
 #pragma unroll
@@ -166,13 +173,13 @@ __device__ __forceinline__ void mma_part(v16f (&acc)[4][BB], const Frags<BB> &f)
                 const v8i fa = __builtin_bit_cast(v8i, (Bytes32{f.a[ai][0], f.a[ai][1]}));
                 const v8i fb = __builtin_bit_cast(v8i, (Bytes32{f.b[bj][0], f.b[bj][1]}));
                 acc[ai][bj] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                    fa, fb, acc[ai][bj], 0, 0, 0, 0, 0, 0);
+                    fa, fb, FIRST ? zero : acc[ai][bj], 0, 0, 0, 0, 0, 0);
             } else {
 #pragma unroll
                 for (int kk = 0; kk < 2; ++kk)
                     acc[ai][bj] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
                         __builtin_bit_cast(v8h, f.a[ai][kk]), __builtin_bit_cast(v8h, f.b[bj][kk]),
-                        acc[ai][bj], 0, 0, 0);
+                        (FIRST && kk == 0) ? zero : acc[ai][bj], 0, 0, 0);
             }
         }
 }
@@ -191,19 +198,23 @@ __device__ __forceinline__ void tile_product(v16f (&acc)[4][BB], const GemmArgs 
     ts.q = rows_rsrc(a.q, r0, a.M, Tile<WR, BB>::BR, a.ldq);
     ts.ldb = (int)a.ldb;
     ts.ldq = (int)a.ldq;
-#pragma unroll
-    for (int ai = 0; ai < 4; ++ai)
-#pragma unroll
-        for (int bj = 0; bj < BB; ++bj)
-#pragma unroll
-            for (int i = 0; i < 16; ++i) acc[ai][bj][i] = 0.0f;
-    const int nk = a.Kb / kBK;
+    const int nk = a.Kb / kBK;  // >= 1
     StageRegs<WR, BB> g;
     Frags<BB> f;
     g_load<WR, BB>(g, ts, tid, 0);
     s_store<WR, BB>(g, sb0, tid);
     if (nk > 1) g_load<WR, BB>(g, ts, tid, kBK);
-    for (int s = 0; s < nk; ++s) {
+    {  // stage 0 accumulates onto zero
+        __syncthreads();
+        ld_frags_b<ELEM, BB>(f, sb0, wr, lane);
+        ld_frags_a<ELEM, BB, 0, 2>(f, sb0, wi, lane);
+        if (1 < nk) s_store<WR, BB>(g, sb1, tid);
+        if (2 < nk) g_load<WR, BB>(g, ts, tid, 2 * kBK);
+        mma_part<ELEM, BB, 0, 2, true>(acc, f);
+        ld_frags_a<ELEM, BB, 2, 4>(f, sb0, wi, lane);
+        mma_part<ELEM, BB, 2, 4, true>(acc, f);
+    }
+    for (int s = 1; s < nk; ++s) {
         uint8_t *cur = (s & 1) ? sb1 : sb0;
         uint8_t *nxt = (s & 1) ? sb0 : sb1;
         __syncthreads();
