@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-fasta", action="store_true")
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="skip timing hipBLASLt's GEMM at the same M and K")
     ap.add_argument("--serve-rates", default=None,
                     help="Poisson arrival rates, req/s (100mbp: 50,200,1000; human: 10,50,200,1000)")
     ap.add_argument("--serve-requests", default=None,
@@ -275,6 +277,39 @@ def main():
     }
     del eng16
     torch.cuda.empty_cache()
+
+    # ---- the vendor GEMM at the same M and K: hipBLASLt through torch, plain C write
+    if not args.no_vendor:
+        nv = int(min(N, 98_304))
+        ga = (torch.randn(M, 2048, device=dev) / 8).to(torch.float8_e4m3fn)
+        gb = (torch.randn(nv, 2048, device=dev) / 8).to(torch.float8_e4m3fn)
+        one = torch.ones((), device=dev)
+
+        def vt(fn, reps):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        fl = 2.0 * M * nv * 2048
+        ms8 = vt(lambda: torch._scaled_mm(ga, gb.t(), scale_a=one, scale_b=one,
+                                          out_dtype=torch.bfloat16), 10)
+        ga16, gb16 = ga.to(torch.float16), gb.to(torch.float16)
+        ms16 = vt(lambda: ga16 @ gb16.t(), 5)
+        v8, v16 = fl / ms8 / 1e9, fl / ms16 / 1e9
+        result["vendor_gemm"] = {
+            "library": "hipBLASLt via torch._scaled_mm (e4m3fn, bf16 C) and torch.matmul (fp16 C)",
+            "shape_mnk": [M, nv, 2048], "fp8_tflops": round(v8, 1), "fp16_tflops": round(v16, 1),
+            "fused_fp8_over_vendor_fp8": round(achieved / v8, 3),
+            "fused_fp16_over_vendor_fp16": round(result["roofline_fp16"]["achieved"] / v16, 3)}
+        log(f"vendor GEMM at M={M} N={nv} K=2048: fp8 {v8:.0f} TFLOP/s, fp16 {v16:.0f} TFLOP/s")
+        del ga, gb, ga16, gb16
+        torch.cuda.empty_cache()
 
     # ---- FASTA ingestion: the batch as FASTA text, parsed and packed on the GPU
     if not args.no_fasta:

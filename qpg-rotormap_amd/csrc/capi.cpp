@@ -77,11 +77,21 @@ int table_get(rm_ctx *ctx, int k, int s, int m, const int32_t **out) {
 // kernel holds a CU's LDS) -- rounding up would leave a second, mostly idle
 // wave -- never more splits than column tiles; the autotuned plan overrides
 // per M bucket.
-int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp) {
-    int64_t R = cdiv(M, rows_per_tile(kp)), tiles = cdiv(N, kBI);
-    int64_t want = std::max<int64_t>(1, (int64_t)ctx->num_cus / R);
-    int64_t S = std::max<int64_t>(1, std::min<int64_t>(want, tiles));
-    return (int)std::min<int64_t>(S, 256);
+// Splits of a batch with no tuned plan: at least enough items to give every
+// CU one (S >= CUs / R), then doubled until the query tiles worked on at once
+// -- min(R, CUs / S) tiles of BR rows x row_bytes -- fit in half of the
+// 256 MB MALL.  Each workgroup re-reads its query tile once per column tile;
+// a working set past the MALL sends those re-reads to HBM (65,536 fp16
+// queries of 4 KB: 256 MB at S = 1, 19 % slower than S = 2 at 3.1 M columns;
+// the e4m3 batch, 128 MB, is fastest at S = 1).
+int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp, int64_t row_bytes) {
+    const int64_t BR = rows_per_tile(kp);
+    const int64_t R = cdiv(M, BR), tiles = cdiv(N, kBI), cus = ctx->num_cus;
+    const int64_t cap = std::min<int64_t>(256, tiles);
+    int64_t S = std::max<int64_t>(1, std::min<int64_t>(cus / R, cap));
+    while (S < cap && std::min<int64_t>(R, cdiv(cus, S)) * BR * row_bytes > kQueryWorkingSet)
+        S = std::min<int64_t>(2 * S, cap);
+    return (int)S;
 }
 
 static int bucket_of(int64_t M) {
@@ -91,14 +101,14 @@ static int bucket_of(int64_t M) {
 }
 
 // Tile plan for a batch: (splits, query tiles per XCD group).  The built-in
-// rule fills the CUs with one workgroup each and groups 4 query tiles per
-// XCD; the autotuned plan (rm_autotune / rm_plan_set) overrides per bucket.
+// rule is default_splits with one group over all query tiles; the autotuned
+// plan (rm_autotune / rm_plan_set) overrides it per size bucket.
 std::pair<int, int> plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
     auto it = ctx->plan.find({mode == RM_MODE_FP16 ? 1 : 0, bucket_of(M)});
     int64_t tiles = cdiv(idx->N, kBI);
     if (it != ctx->plan.end())
         return {(int)std::max<int64_t>(1, std::min<int64_t>(it->second.first, tiles)), it->second.second};
-    return {default_splits(ctx, M, idx->N, kp), 0};
+    return {default_splits(ctx, M, idx->N, kp, idx->rdim * (mode == RM_MODE_FP16 ? 2 : 1)), 0};
 }
 
 static int splits_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {

@@ -58,6 +58,9 @@ constexpr int kMaxKP = 64;     // largest top-k' list
 // Queries per tile of the fused kernel: 256 while the per-query lists fit
 // LDS beside the stage buffers (k' <= 32), else 128.
 inline int rows_per_tile(int kp) { return kp <= 32 ? 256 : 128; }
+// query bytes the fused kernel's concurrent workgroups may re-read from the
+// MALL (half of MI355X's 256 MB Infinity Cache)
+constexpr int64_t kQueryWorkingSet = 128LL << 20;
 // LDS layout of the fused kernel for br queries per tile and list length kp:
 // [ stage buffers x2 ][ lists br x (kp+1) x 8 ][ candidate slots br x (capd+1)
 // x 8 ][ tau br ][ slot counters br ][ 3 vote flags ].  The candidate slots
@@ -181,7 +184,7 @@ namespace rm {
 int ws_get(rm_ctx *ctx, const char *name, size_t bytes, void **out);
 int table_get(rm_ctx *ctx, int k, int s, int m, const int32_t **out);
 hipStream_t pick_stream(rm_ctx *ctx, void *stream);
-int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp);
+int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp, int64_t row_bytes);
 // the whole fused query path on already-encoded queries
 int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
               const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,

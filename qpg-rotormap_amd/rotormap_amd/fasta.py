@@ -16,11 +16,14 @@ from . import _lib as L
 
 
 def _as_bytes(data):
+    """A writable contiguous uint8 view (a copy only when the input is read-only)."""
     if isinstance(data, (bytes, bytearray, memoryview)):
-        return np.frombuffer(data, np.uint8)
-    if isinstance(data, np.ndarray):
-        return np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
-    return np.fromfile(data, np.uint8)
+        a = np.frombuffer(data, np.uint8)
+    elif isinstance(data, np.ndarray):
+        a = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+    else:
+        return np.fromfile(data, np.uint8)
+    return a if a.flags.writeable else a.copy()
 
 
 def fasta_reads_gpu(data, k, device=0, stream=None):
@@ -67,7 +70,8 @@ def iter_fasta_gpu(path, k, batch_size=1 << 16, chunk_bytes=1 << 30, device=0):
     carry = np.zeros(0, np.uint8)
     with open(path, "rb") as f:
         while True:
-            chunk = np.frombuffer(f.read(chunk_bytes), np.uint8)
+            chunk = np.empty(chunk_bytes, np.uint8)
+            chunk = chunk[:f.readinto(chunk)]
             eof = chunk.size == 0
             buf = np.concatenate([carry, chunk]) if carry.size else chunk
             if eof:

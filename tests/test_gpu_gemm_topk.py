@@ -253,3 +253,36 @@ def test_index_past_2_32_elements(torch_gpu):
         np.testing.assert_array_equal(l.cpu().numpy()[:, 0], want, err_msg=mode)
         del eng
         torch.cuda.empty_cache()
+
+
+def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
+    """With no tuned plan, the split count is the smallest (from 'one item
+    per CU' up, doubling) for which the query tiles worked on at once,
+    min(R, CUs / S) tiles of 256 rows, fit in 128 MB: a 65,536-read e4m3 batch
+    (2 KB rows, 128 MB) stays unsplit on 256 CUs; the same batch in fp16 (4 KB
+    rows, 256 MB) takes two splits; a 4,096-read batch takes CUs / 16."""
+    torch = torch_gpu
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    idx = R.ResidentIndex(np.zeros((512 * 256, 2048), np.uint8), fp16=True)
+    ctx = L.ctx(0)
+    lib = R.lib()
+    try:
+        for mode in (L.RM_MODE_FP8, L.RM_MODE_FP8_RESCORE, L.RM_MODE_FP16):
+            for M in (65_536, 4_096):
+                L.check(lib.rm_plan_set(ctx, M, mode, 0, 0))
+
+        def expect(M, row_bytes):
+            rt = -(-M // 256)
+            S = max(1, min(cus // rt, 256))
+            while S < 256 and min(rt, -(-cus // S)) * 256 * row_bytes > 128 << 20:
+                S = min(2 * S, 256)
+            return S
+
+        s8 = lib.rm_plan_splits(ctx, idx.handle, 65_536, L.RM_MODE_FP8)
+        s16 = lib.rm_plan_splits(ctx, idx.handle, 65_536, L.RM_MODE_FP16)
+        assert (s8, s16) == (expect(65_536, 2048), expect(65_536, 4096))
+        if cus == 256:
+            assert (s8, s16) == (1, 2)
+        assert lib.rm_plan_splits(ctx, idx.handle, 4_096, L.RM_MODE_FP8) == expect(4_096, 2048)
+    finally:
+        idx.close()
