@@ -19,18 +19,23 @@
 // in 2 x 4, wave tile 128 index rows x 64 queries = 4 x 2 blocks of
 // v_mfma_scale_f32_32x32x64_f8f6f4 (unit scales) per 64-byte K stage, or
 // 2 x v_mfma_f32_32x32x16_f16 per block for the FP16 path; lists longer than
-// 32 use 4 waves over 256 x 128 tiles.  K is streamed through double-buffered
-// LDS in 64-byte stages (64-byte rows, 16-byte chunks XOR-swizzled by
-// (row >> 2) & 3, conflict-free for the fragment reads), global -> register
-// loads two stages ahead, register -> LDS stores one stage ahead, one barrier
-// per stage.  A workgroup owns one query tile and walks a contiguous range
-// of index tiles (its column split); per query it keeps a top-k' list in LDS
-// and a threshold tau = the list's k'-th value.  After each tile a lane flags
+// 32 use 4 waves over 256 x 128 tiles.  K is streamed through LDS in 64-byte
+// stages (64-byte rows, 16-byte chunks XOR-swizzled by (row >> 2) & 3,
+// conflict-free for the fragment reads), one barrier per stage.  The
+// 256-query tile fills a ring of four stage buffers with global -> LDS loads
+// three stages ahead (no staging registers, no LDS store instructions); the
+// 128-query tile and the C-writing kernel double-buffer through registers
+// (global -> register loads two stages ahead, register -> LDS stores one
+// stage ahead).  A workgroup owns one query tile and walks a contiguous range
+// of index tiles (its column split); per query it keeps a top-k' list and a
+// threshold tau = the list's k'-th value.  After each tile a lane flags
 // values > tau (a query's 16 x 4 values are compared one by one only when
 // their maximum beats tau); flagged values are appended to per-query LDS
 // slots that persist across tiles and are merged into the lists (one thread
 // per query, (score desc, column asc) order) only when some query's slots
-// overflow.  The grid is persistent by default (one workgroup per CU walking
+// overflow.  With the ring the lists live in the split partials and are
+// staged into the idle ring for a merge; otherwise they stay in LDS.  The
+// grid is persistent by default (one workgroup per CU walking
 // its XCD's items); items are ordered so that the concurrently resident
 // workgroups of one XCD (dispatch is round-robin over the 8 XCDs) share
 // index tiles through that XCD's L2.
@@ -228,6 +233,102 @@ __device__ __forceinline__ void tile_product(v16f (&acc)[4][BB], const GemmArgs 
     }
 }
 
+// ---------------------------------------------------------------------------
+// K loop of the 8-wave, 256-query tile fed by global -> LDS loads into a ring
+// of kRing stage buffers (kRing - 1 stages ahead).  No staging registers and
+// no LDS store instructions: each wave loads 64 of the stage's 512 rows
+// (waves 0-3 index rows, 4-7 query rows) as 4 x 16-row pieces of 1 KB, lane
+// l writing LDS bytes l*16 of its piece; the lane fetches the logical chunk
+// that the 64-byte swizzle puts there.  A stage waits for its own loads
+// (vmcnt counts this wave's 4 loads per stage) before the barrier that
+// publishes them, and reuses the buffer of the stage before it.
+// ---------------------------------------------------------------------------
+static_assert(!kRingOk || kSwz == 1, "ring rows use the 64-byte swizzle");
+
+struct RingSrc {
+    __amdgpu_buffer_rsrc_t r;  // this wave's operand
+    int voff;                  // lane offset of its first 16-row piece
+    int step16;                // bytes between pieces (16 rows)
+    int lds_off;               // LDS offset of the wave's rows in a stage buffer
+};
+
+__device__ __forceinline__ RingSrc ring_src(const TileSrc &ts, int tid) {
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: no waterfall
+    const int lane = tid & 63;
+    // LDS row (wid*64 + piece*16 + lane/4) has ((row >> 2) & 3) = (lane >> 4) & 3
+    const int c = (lane & 3) ^ ((lane >> 4) & 3);
+    const int ld = wid < 4 ? ts.ldb : ts.ldq;
+    RingSrc d;
+    d.r = wid < 4 ? ts.db : ts.q;
+    d.voff = ((wid & 3) * 64 + (lane >> 2)) * ld + c * 16;
+    d.step16 = 16 * ld;
+    d.lds_off = wid * 64 * kRow;
+    return d;
+}
+
+__device__ __forceinline__ void ring_load(const RingSrc &d, uint8_t *sb, int kb) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            d.r, (__attribute__((address_space(3))) void *)(sb + d.lds_off + j * 16 * kRow), 16,
+            d.voff, kb + j * d.step16, 0, 0);
+}
+
+// One stage: wait until stage s (issued kRing - 1 stages ago) has landed in
+// every wave's rows, then load stage s + kRing - 1 into the buffer stage s - 1
+// left and run stage s's MFMAs.  The restrict pointers give the fragment
+// reads an alias scope apart from the loads, so the compiler does not wait
+// for the loads in flight before each read; the barrier is written out so
+// that its wait stays vmcnt(8) (two younger stages in flight) instead of the
+// vmcnt(0) a fence would add.
+template <int ELEM, bool FIRST>
+__device__ __forceinline__ void ring_stage(v16f (&acc)[4][2], const RingSrc &d,
+                                           uint8_t *__restrict__ bw, const uint8_t *__restrict__ br,
+                                           int s, int nk, int wi, int wr, int lane) {
+    static_assert(kRing == 4, "vmcnt below counts kRing - 2 stages of 4 loads");
+    asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    Frags<2> f;
+    ld_frags_b<ELEM, 2>(f, br, wr, lane);
+    ld_frags_a<ELEM, 2, 0, 2>(f, br, wi, lane);
+    ring_load(d, bw, min(s + kRing - 1, nk - 1) * kBK);  // past the end: an unread buffer
+    mma_part<ELEM, 2, 0, 2, FIRST>(acc, f);
+    ld_frags_a<ELEM, 2, 2, 4>(f, br, wi, lane);
+    mma_part<ELEM, 2, 2, 4, FIRST>(acc, f);
+}
+
+template <int ELEM>
+__device__ __forceinline__ void tile_product_ring(v16f (&acc)[4][2], const GemmArgs &a, int64_t i0,
+                                                  int64_t r0, uint8_t *ring, int tid, int wi,
+                                                  int wr, int lane) {
+    TileSrc ts;
+    ts.db = rows_rsrc(a.db, i0, a.N, kBI, a.ldb);
+    ts.q = rows_rsrc(a.q, r0, a.M, Tile<4, 2>::BR, a.ldq);
+    ts.ldb = (int)a.ldb;
+    ts.ldq = (int)a.ldq;
+    const RingSrc d = ring_src(ts, tid);
+    const int nk = a.Kb / kBK;  // >= 1
+    constexpr int SB = Tile<4, 2>::STAGE_BYTES;
+    uint8_t *b0 = ring, *b1 = ring + SB, *b2 = ring + 2 * SB, *b3 = ring + 3 * SB;
+    // the callers' last barrier ordered every earlier use of the ring
+    ring_load(d, b0, 0);
+    ring_load(d, b1, min(1, nk - 1) * kBK);
+    ring_load(d, b2, min(2, nk - 1) * kBK);
+    ring_stage<ELEM, true>(acc, d, b3, b0, 0, nk, wi, wr, lane);
+    int s = 1;
+    for (; s + 3 < nk; s += 4) {
+        ring_stage<ELEM, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
+        ring_stage<ELEM, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
+        ring_stage<ELEM, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
+        ring_stage<ELEM, false>(acc, d, b3, b0, s + 3, nk, wi, wr, lane);
+    }
+    if (s < nk) ring_stage<ELEM, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
+    if (s + 1 < nk) ring_stage<ELEM, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
+    if (s + 2 < nk) ring_stage<ELEM, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
+    // every load of the tile (the clamped ones too) has landed and every
+    // fragment read is done before the epilogue reuses the ring
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // Float <-> int key with the same order (signed int compare): negative
 // floats get their magnitude bits flipped.
 __device__ __forceinline__ int order_key(float f) {
@@ -347,6 +448,46 @@ __device__ __forceinline__ void merge_slots(float *lv, int32_t *lc, const float 
     }
 }
 
+// Ring layout: a (query, split) list lives in the split partials between
+// merges.  For a merge it is staged into the idle ring (LDS rows of LP
+// entries): from the partials once a merge has stored it, else empty.  One
+// thread per query, the same thread that stores it back, so no other thread
+// reads a list between its store and its next load.
+template <int BR, int NT>
+__device__ __forceinline__ void stage_lists(float *lv, int32_t *lc, const GemmArgs &a, int64_t r0,
+                                            int split, int KP, int LP, bool stored, int tid) {
+    for (int q = tid; q < BR; q += NT) {
+        float *L = lv + q * LP;
+        int32_t *Lc = lc + q * LP;
+        if (stored && r0 + q < a.M) {
+            const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
+            for (int j = 0; j < KP; ++j) {
+                L[j] = a.part_v[base + j];
+                Lc[j] = a.part_c[base + j];
+            }
+        } else {
+            for (int j = 0; j < KP; ++j) {
+                L[j] = -INFINITY;
+                Lc[j] = INT_MAX;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int BR, int NT>
+__device__ __forceinline__ void store_lists(const float *lv, const int32_t *lc, const GemmArgs &a,
+                                            int64_t r0, int split, int KP, int LP, int tid) {
+    for (int q = tid; q < BR; q += NT) {
+        if (r0 + q >= a.M) continue;
+        const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
+        for (int j = 0; j < KP; ++j) {
+            a.part_v[base + j] = lv[q * LP + j];
+            a.part_c[base + j] = lc[q * LP + j];
+        }
+    }
+}
+
 template <int ELEM, int WR, int BB>
 __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int split) {
     constexpr int BR = Tile<WR, BB>::BR;
@@ -355,10 +496,14 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid & 1, wr = wid >> 1;
     const int KP = a.KP, LP = KP + 1, capd = a.capd, CP = capd + 1;
     static_assert(2 * Tile<WR, BB>::STAGE_BYTES == stage_pair_bytes(BR), "stage layout");
+    // RING: the 8-wave tile streams K through the ring and keeps the lists in
+    // the split partials, staged at the ring's start for merges
+    constexpr bool RING = kRingOk && WR == 4 && BB == 2;
+    static_assert(!RING || kRing * Tile<WR, BB>::STAGE_BYTES == ring_bytes(BR), "ring layout");
     uint8_t *sb0 = smem, *sb1 = smem + Tile<WR, BB>::STAGE_BYTES;
-    float *lv = (float *)(smem + list_offset(BR));       // BR x (KP+1)
+    float *lv = (float *)(smem + (RING ? 0 : list_offset(BR)));  // BR x (KP+1)
     int32_t *lc = (int32_t *)(lv + BR * LP);
-    float *cv = (float *)(smem + cand_offset(BR, KP));   // BR x (capd+1)
+    float *cv = (float *)(smem + (RING ? ring_cand_offset(BR) : cand_offset(BR, KP)));  // BR x (capd+1)
     int32_t *cc = (int32_t *)(cv + BR * CP);
     float *tauv = (float *)(cc + BR * CP);               // BR
     int *cnt = (int *)(tauv + BR);                       // BR
@@ -367,7 +512,11 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     const int64_t r0 = rt * BR;
     const int64_t t0 = (int64_t)split * a.tiles / a.S, t1 = (int64_t)(split + 1) * a.tiles / a.S;
 
-    for (int i = tid; i < BR * LP; i += NT) { lv[i] = -INFINITY; lc[i] = INT_MAX; }
+    if constexpr (!RING)
+        for (int i = tid; i < BR * LP; i += NT) { lv[i] = -INFINITY; lc[i] = INT_MAX; }
+    // RING: `stored` = the lists in the partials hold this item's merges so
+    // far (else they are empty); `staged` = they sit in LDS for this tile
+    bool stored = false, staged = false;
     for (int i = tid; i < BR; i += NT) { tauv[i] = -INFINITY; cnt[i] = 0; }
     if (tid < 3) flags[tid] = 0;
     int vote = 0;
@@ -386,7 +535,10 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     v16f acc[4][BB];
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
-        tile_product<ELEM, WR, BB>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
+        if constexpr (RING)
+            tile_product_ring<ELEM>(acc, a, i0, r0, smem, tid, wi, wr, lane);
+        else
+            tile_product<ELEM, WR, BB>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
         // columns of this tile valid for this lane: RC(ai, i) < lim
         const int lim = (int)min<int64_t>(a.N - i0 - rowbase, (int64_t)kBI);
         const int32_t colbase = (int32_t)(a.col_base + i0 + rowbase);
@@ -502,6 +654,10 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
             // slots of every query still hold what they took: merge only when
             // some value found its query's slots full
             if (!block_any(any != 0ull, flags, vote)) break;
+            if constexpr (RING) {
+                if (!staged) stage_lists<BR, NT>(lv, lc, a, r0, split, KP, LP, stored, tid);
+                staged = true;
+            }
             merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, LP, CP, capd, tid, NT);
             __syncthreads();
             // --- overflowed values retry against the raised tau (>=: an equal
@@ -523,19 +679,22 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                 any |= pend[bj];
             }
         }
+        if constexpr (RING) {
+            // the next tile's loads refill the ring: lists go back first
+            if (staged) {
+                store_lists<BR, NT>(lv, lc, a, r0, split, KP, LP, tid);
+                stored = true;
+                staged = false;
+                __syncthreads();
+            }
+        }
     }
 #undef RC
     __syncthreads();
+    if constexpr (RING) stage_lists<BR, NT>(lv, lc, a, r0, split, KP, LP, stored, tid);
     merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, LP, CP, capd, tid, NT);
     __syncthreads();
-    for (int q = tid; q < BR; q += NT) {
-        if (r0 + q >= a.M) continue;
-        const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
-        for (int j = 0; j < KP; ++j) {
-            a.part_v[base + j] = lv[q * LP + j];
-            a.part_c[base + j] = lc[q * LP + j];
-        }
-    }
+    store_lists<BR, NT>(lv, lc, a, r0, split, KP, LP, tid);
 }
 
 template <int ELEM, int WR, int BB>
@@ -545,7 +704,7 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
         int split;
         if (!decode_item(a, it, a.items8, rt, split)) continue;
         topk_item<ELEM, WR, BB>(a, rt, split);
-        __syncthreads();  // the next item re-initialises the lists in LDS
+        __syncthreads();  // the next item re-initialises LDS (and refills the ring)
     }
 }
 
@@ -565,12 +724,12 @@ static int launch_common(const GemmArgs &a0, const void *fn, size_t lds, int nt,
 }
 
 // Kernel shapes: 8 waves (two per SIMD) with 128 x 64 wave tiles over
-// 256 x 256 tiles; lists longer than 32 use 4 waves x 128 x 64 (128 queries
-// per tile).
+// 256 x 256 tiles, fed by the load ring; lists longer than 32 use 4 waves x
+// 128 x 64 (128 queries per tile) with register staging.
 
 int launch_gemm_topk(const GemmArgs &a0, int elem, hipStream_t st) {
     GemmArgs a = a0;
-    a.capd = cand_slots(rows_per_tile(a.KP), a.KP);
+    a.capd = topk_cand_slots(a.KP);
     RM_CHECK(a.capd >= 1, RM_ERR_SHAPE, "no LDS left for candidate slots (k' = %d)", a.KP);
     const void *fn;
     int nt = 256;

@@ -61,8 +61,8 @@ inline int rows_per_tile(int kp) { return kp <= 32 ? 256 : 128; }
 // query bytes the fused kernel's concurrent workgroups may re-read from the
 // MALL (half of MI355X's 256 MB Infinity Cache)
 constexpr int64_t kQueryWorkingSet = 128LL << 20;
-// LDS layout of the fused kernel for br queries per tile and list length kp:
-// [ stage buffers x2 ][ lists br x (kp+1) x 8 ][ candidate slots br x (capd+1)
+// LDS layout of the fused kernel without the ring (br queries per tile, list
+// length kp): [ stage buffers x2 ][ lists br x (kp+1) x 8 ][ candidate slots br x (capd+1)
 // x 8 ][ tau br ][ slot counters br ][ 3 vote flags ].  The candidate slots
 // persist across tiles (merged into the lists only when some query's slots
 // overflow); capd takes what is left of the 160 KiB, at most 64.
@@ -79,7 +79,33 @@ constexpr int cand_slots(int br, int kp) {
 constexpr size_t topk_lds_bytes_for(int br, int kp) {
     return cand_offset(br, kp) + (size_t)br * (cand_slots(br, kp) + 1) * 8 + tail_bytes(br);
 }
-inline size_t topk_lds_bytes(int kp) { return topk_lds_bytes_for(rows_per_tile(kp), kp); }
+// The 256-query tile (k' <= 32) streams K through a ring of kRing stage
+// buffers filled by global -> LDS loads, and keeps each query's list in the
+// split partials (global memory), staged into the idle ring for merges:
+// [ ring kRing x (kBI + br) x kRow ][ candidate slots br x (capd+1) x 8 ][ tau
+// br ][ slot counters br ][ 3 vote flags ].  The ring's load rows follow the
+// 64-byte swizzled row layout.
+constexpr int kRing = 4;
+constexpr bool kRingOk = kSwz == 1;
+inline bool uses_ring(int kp) { return kRingOk && rows_per_tile(kp) == 256; }
+constexpr size_t ring_bytes(int br) { return (size_t)kRing * (kBI + br) * kRow; }
+constexpr size_t ring_cand_offset(int br) { return ring_bytes(br); }
+constexpr int ring_cand_slots(int br) {
+    const size_t used = ring_cand_offset(br) + tail_bytes(br);
+    const long long avail = used >= kLdsMax ? 0 : (long long)((kLdsMax - used) / ((size_t)br * 8)) - 1;
+    return avail > 64 ? 64 : (int)avail;
+}
+constexpr size_t ring_lds_bytes(int br) {
+    return ring_cand_offset(br) + (size_t)br * (ring_cand_slots(br) + 1) * 8 + tail_bytes(br);
+}
+// lists staged for a merge: br x (kp+1) values, then as many columns
+constexpr size_t ring_staged_list_bytes(int br, int kp) { return (size_t)br * (kp + 1) * 8; }
+inline int topk_cand_slots(int kp) {
+    return uses_ring(kp) ? ring_cand_slots(256) : cand_slots(rows_per_tile(kp), kp);
+}
+inline size_t topk_lds_bytes(int kp) {
+    return uses_ring(kp) ? ring_lds_bytes(256) : topk_lds_bytes_for(rows_per_tile(kp), kp);
+}
 inline size_t store_lds_bytes() { return 2 * (size_t)(kBI + 256) * kRow; }
 
 struct GemmArgs {

@@ -1,6 +1,7 @@
 // Host-side check of the fused kernel's LDS layout (rm_internal.h): for every
 // list length and stage-buffer layout the candidate slots, the lists and the
-// per-query words are disjoint and inside the LDS the launch requests.
+// per-query words are disjoint and inside the LDS the launch requests (for
+// the ring layout: the staged lists fit the ring).
 #include <cstdio>
 
 #include "../qpg-rotormap_amd/csrc/rm_internal.h"
@@ -9,6 +10,23 @@ int main() {
     int bad = 0;
     for (int kp = 1; kp <= rm::kMaxKP; ++kp) {
         const int br = rm::rows_per_tile(kp);
+        if (rm::uses_ring(kp)) {
+            // ring layout: the ring, then candidate slots, tau, counters, 3
+            // flags; a merge stages the lists (and the first tile its seed
+            // words) inside the ring
+            const size_t total = rm::topk_lds_bytes(kp);
+            const int capd = rm::topk_cand_slots(kp);
+            const size_t cand_end = rm::ring_cand_offset(br) + (size_t)br * (capd + 1) * 8;
+            const size_t end = cand_end + (size_t)br * 8 + 3 * 4;
+            if (capd < 1 || rm::ring_cand_offset(br) < rm::ring_bytes(br) || end > total ||
+                total > rm::kLdsMax || rm::ring_staged_list_bytes(br, kp) > rm::ring_bytes(br) ||
+                (size_t)br * 4 > rm::ring_bytes(br)) {
+                std::printf("ring kp=%d br=%d capd=%d: cands@%zu end %zu total %zu\n", kp, br, capd,
+                            rm::ring_cand_offset(br), end, total);
+                ++bad;
+            }
+            continue;
+        }
         const size_t total = rm::topk_lds_bytes(kp);
         const int capd = rm::cand_slots(br, kp);
         // regions in kernel order: stage buffers, lists, candidate slots, tau,
