@@ -296,6 +296,23 @@ __device__ __forceinline__ void ring_stage(v16f (&acc)[4][2], const RingSrc &d,
     mma_part<ELEM, 2, 2, 4, FIRST>(acc, f);
 }
 
+// A stage of the streamed loop that loads stage `kb` of `d` (the next tile)
+// into bw, or (LOAD false) loads nothing.
+template <int ELEM, bool LOAD>
+__device__ __forceinline__ void ring_stage_next(v16f (&acc)[4][2], const RingSrc &d,
+                                                uint8_t *__restrict__ bw,
+                                                const uint8_t *__restrict__ br, int kb, int wi,
+                                                int wr, int lane) {
+    asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    Frags<2> f;
+    ld_frags_b<ELEM, 2>(f, br, wr, lane);
+    ld_frags_a<ELEM, 2, 0, 2>(f, br, wi, lane);
+    if constexpr (LOAD) ring_load(d, bw, kb);
+    mma_part<ELEM, 2, 0, 2, false>(acc, f);
+    ld_frags_a<ELEM, 2, 2, 4>(f, br, wi, lane);
+    mma_part<ELEM, 2, 2, 4, false>(acc, f);
+}
+
 template <int ELEM>
 __device__ __forceinline__ void tile_product_ring(v16f (&acc)[4][2], const GemmArgs &a, int64_t i0,
                                                   int64_t r0, uint8_t *ring, int tid, int wi,
@@ -327,6 +344,60 @@ __device__ __forceinline__ void tile_product_ring(v16f (&acc)[4][2], const GemmA
     // every load of the tile (the clamped ones too) has landed and every
     // fragment read is done before the epilogue reuses the ring
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// The same K loop as one stream over the tiles of an item (K a multiple of
+// 4 stages): the last stages of a tile load the next tile's stages 0 and 1,
+// and the next tile loads its stage 2 on entry, so no tile waits for its
+// first stage and no load is wasted.  During the epilogue the next tile's
+// stages 0-1 may be in flight into buffers 0-1; buffers 2-3 are idle (the
+// epilogue's scratch).  The last tile of an item loads its last stage again
+// in place of the next tile's (into buffers 0-1, which the next item's
+// first tile refills after the item's closing barrier) so that every stage
+// keeps two younger stages in flight for the vmcnt(8) wait.
+template <int ELEM>
+__device__ __forceinline__ void tile_product_ring_stream(v16f (&acc)[4][2], const GemmArgs &a,
+                                                         int64_t i0, int64_t r0, uint8_t *ring,
+                                                         int tid, int wi, int wr, int lane,
+                                                         bool first, bool has_next) {
+    TileSrc ts;
+    ts.db = rows_rsrc(a.db, i0, a.N, kBI, a.ldb);
+    ts.q = rows_rsrc(a.q, r0, a.M, Tile<4, 2>::BR, a.ldq);
+    ts.ldb = (int)a.ldb;
+    ts.ldq = (int)a.ldq;
+    const RingSrc d = ring_src(ts, tid);
+    const int nk = a.Kb / kBK;  // a multiple of 4
+    RingSrc dn = d;             // next tile (or this one again)
+    int kn = (nk - 1) * kBK;    // next tile's stage 0 (or this tile's last)
+    if (has_next) {
+        ts.db = rows_rsrc(a.db, i0 + kBI, a.N, kBI, a.ldb);
+        dn = ring_src(ts, tid);
+        kn = 0;
+    }
+    constexpr int SB = Tile<4, 2>::STAGE_BYTES;
+    uint8_t *b0 = ring, *b1 = ring + SB, *b2 = ring + 2 * SB, *b3 = ring + 3 * SB;
+    // the callers' last barrier ordered every earlier use of buffers 2-3
+    // (and of 0-1 on an item's first tile; this wave's loads into its rows of
+    // 0-1 from the previous item have landed)
+    if (first) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        ring_load(d, b0, 0);
+        ring_load(d, b1, kBK);
+    }
+    ring_load(d, b2, 2 * kBK);
+    ring_stage<ELEM, true>(acc, d, b3, b0, 0, nk, wi, wr, lane);
+    for (int s = 1; s + 3 < nk; s += 4) {
+        ring_stage<ELEM, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
+        ring_stage<ELEM, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
+        ring_stage<ELEM, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
+        ring_stage<ELEM, false>(acc, d, b3, b0, s + 3, nk, wi, wr, lane);
+    }
+    // stages nk-3 .. nk-1 (buffers 1, 2, 3) load the next tile's 0 and 1
+    ring_stage_next<ELEM, true>(acc, dn, b0, b1, kn, wi, wr, lane);
+    ring_stage_next<ELEM, true>(acc, dn, b1, b2, kn + (has_next ? kBK : 0), wi, wr, lane);
+    ring_stage_next<ELEM, false>(acc, dn, b2, b3, 0, wi, wr, lane);
+    // every fragment read is done before the epilogue uses buffers 2-3
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 // Float <-> int key with the same order (signed int compare): negative
@@ -420,70 +491,82 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_store_kernel(GemmArgs a) {
     }
 }
 
+// LDS index of entry j of query q's list: rows of LP = k' + 1 entries, or
+// (ROT, lists staged in the ring, k' <= 32) rows of 32 entries rotated by q,
+// so that a warp-wide step of the one-thread-per-query merge (same j,
+// consecutive q) touches 64 different banks.
+template <bool ROT>
+struct ListAt {
+    int LP;
+    __device__ __forceinline__ int operator()(int q, int j) const {
+        return ROT ? q * 32 + ((j + q) & 31) : q * LP + j;
+    }
+};
+
 // Merge every query's candidate slots into its sorted list (one thread per
 // query), reset the slot counters and raise tau to the list's k'-th value.
+template <bool ROT>
 __device__ __forceinline__ void merge_slots(float *lv, int32_t *lc, const float *cv,
                                             const int32_t *cc, float *tauv, int *cnt, int BR,
-                                            int KP, int LP, int CP, int capd, int tid, int nt) {
+                                            int KP, ListAt<ROT> at, int CP, int capd, int tid,
+                                            int nt) {
     for (int q = tid; q < BR; q += nt) {
         const int nc = min(cnt[q], capd);
         if (nc == 0) continue;
-        float *L = lv + q * LP;
-        int32_t *Lc = lc + q * LP;
         for (int j = 0; j < nc; ++j) {
             const float v = cv[q * CP + j];
             const int32_t c = cc[q * CP + j];
-            if (!beats(v, c, L[KP - 1], Lc[KP - 1])) continue;
+            if (!beats(v, c, lv[at(q, KP - 1)], lc[at(q, KP - 1)])) continue;
             int pos = KP - 1;
-            while (pos > 0 && beats(v, c, L[pos - 1], Lc[pos - 1])) {
-                L[pos] = L[pos - 1];
-                Lc[pos] = Lc[pos - 1];
+            while (pos > 0 && beats(v, c, lv[at(q, pos - 1)], lc[at(q, pos - 1)])) {
+                lv[at(q, pos)] = lv[at(q, pos - 1)];
+                lc[at(q, pos)] = lc[at(q, pos - 1)];
                 --pos;
             }
-            L[pos] = v;
-            Lc[pos] = c;
+            lv[at(q, pos)] = v;
+            lc[at(q, pos)] = c;
         }
         cnt[q] = 0;
-        tauv[q] = L[KP - 1];
+        tauv[q] = lv[at(q, KP - 1)];
     }
 }
 
 // Ring layout: a (query, split) list lives in the split partials between
-// merges.  For a merge it is staged into the idle ring (LDS rows of LP
-// entries): from the partials once a merge has stored it, else empty.  One
-// thread per query, the same thread that stores it back, so no other thread
-// reads a list between its store and its next load.
-template <int BR, int NT>
+// merges.  For a merge it is staged into the two ring buffers that are idle
+// during the epilogue: from the partials once a merge has stored it, else
+// empty.  One thread per query, the same thread that stores it back, so no
+// other thread reads a list between its store and its next load.
+template <int BR, int NT, bool ROT>
 __device__ __forceinline__ void stage_lists(float *lv, int32_t *lc, const GemmArgs &a, int64_t r0,
-                                            int split, int KP, int LP, bool stored, int tid) {
+                                            int split, int KP, ListAt<ROT> at, bool stored,
+                                            int tid) {
     for (int q = tid; q < BR; q += NT) {
-        float *L = lv + q * LP;
-        int32_t *Lc = lc + q * LP;
         if (stored && r0 + q < a.M) {
             const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
             for (int j = 0; j < KP; ++j) {
-                L[j] = a.part_v[base + j];
-                Lc[j] = a.part_c[base + j];
+                lv[at(q, j)] = a.part_v[base + j];
+                lc[at(q, j)] = a.part_c[base + j];
             }
         } else {
             for (int j = 0; j < KP; ++j) {
-                L[j] = -INFINITY;
-                Lc[j] = INT_MAX;
+                lv[at(q, j)] = -INFINITY;
+                lc[at(q, j)] = INT_MAX;
             }
         }
     }
     __syncthreads();
 }
 
-template <int BR, int NT>
+template <int BR, int NT, bool ROT>
 __device__ __forceinline__ void store_lists(const float *lv, const int32_t *lc, const GemmArgs &a,
-                                            int64_t r0, int split, int KP, int LP, int tid) {
+                                            int64_t r0, int split, int KP, ListAt<ROT> at,
+                                            int tid) {
     for (int q = tid; q < BR; q += NT) {
         if (r0 + q >= a.M) continue;
         const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
         for (int j = 0; j < KP; ++j) {
-            a.part_v[base + j] = lv[q * LP + j];
-            a.part_c[base + j] = lc[q * LP + j];
+            a.part_v[base + j] = lv[at(q, j)];
+            a.part_c[base + j] = lc[at(q, j)];
         }
     }
 }
@@ -497,12 +580,18 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     const int KP = a.KP, LP = KP + 1, capd = a.capd, CP = capd + 1;
     static_assert(2 * Tile<WR, BB>::STAGE_BYTES == stage_pair_bytes(BR), "stage layout");
     // RING: the 8-wave tile streams K through the ring and keeps the lists in
-    // the split partials, staged at the ring's start for merges
+    // the split partials; the epilogue's scratch (first-tile seed, lists
+    // staged for a merge: BR x 32 values in ring buffer 2, columns in 3) sits
+    // in the ring buffers no load is in flight to
     constexpr bool RING = kRingOk && WR == 4 && BB == 2;
-    static_assert(!RING || kRing * Tile<WR, BB>::STAGE_BYTES == ring_bytes(BR), "ring layout");
-    uint8_t *sb0 = smem, *sb1 = smem + Tile<WR, BB>::STAGE_BYTES;
-    float *lv = (float *)(smem + (RING ? 0 : list_offset(BR)));  // BR x (KP+1)
-    int32_t *lc = (int32_t *)(lv + BR * LP);
+    constexpr int SB = Tile<WR, BB>::STAGE_BYTES;
+    static_assert(!RING || kRing * SB == ring_bytes(BR), "ring layout");
+    static_assert(!RING || BR * 32 * 4 <= SB, "staged lists fit one ring buffer each");
+    uint8_t *sb0 = smem, *sb1 = smem + SB;
+    uint8_t *scratch = RING ? smem + 2 * SB : sb0;
+    float *lv = (float *)(RING ? scratch : smem + list_offset(BR));  // lists
+    int32_t *lc = (int32_t *)(RING ? scratch + SB : (uint8_t *)(lv + BR * LP));
+    const ListAt<RING> at{LP};
     float *cv = (float *)(smem + (RING ? ring_cand_offset(BR) : cand_offset(BR, KP)));  // BR x (capd+1)
     int32_t *cc = (int32_t *)(cv + BR * CP);
     float *tauv = (float *)(cc + BR * CP);               // BR
@@ -535,9 +624,13 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     v16f acc[4][BB];
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
-        if constexpr (RING)
-            tile_product_ring<ELEM>(acc, a, i0, r0, smem, tid, wi, wr, lane);
-        else
+        if constexpr (RING) {
+            if ((a.Kb / kBK) % 4 == 0)
+                tile_product_ring_stream<ELEM>(acc, a, i0, r0, smem, tid, wi, wr, lane, t == t0,
+                                               t + 1 < t1);
+            else
+                tile_product_ring<ELEM>(acc, a, i0, r0, smem, tid, wi, wr, lane);
+        } else
             tile_product<ELEM, WR, BB>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
         // columns of this tile valid for this lane: RC(ai, i) < lim
         const int lim = (int)min<int64_t>(a.N - i0 - rowbase, (int64_t)kBI);
@@ -580,7 +673,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                 }
                 key[bj] = order_key(bound);
             }
-            int *seed = (int *)sb0;  // the stage buffers are idle until the next tile
+            int *seed = (int *)scratch;  // idle until the next tile
             __syncthreads();
             for (int i = tid; i < BR; i += NT) seed[i] = INT_MAX;
             __syncthreads();
@@ -655,10 +748,10 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
             // some value found its query's slots full
             if (!block_any(any != 0ull, flags, vote)) break;
             if constexpr (RING) {
-                if (!staged) stage_lists<BR, NT>(lv, lc, a, r0, split, KP, LP, stored, tid);
+                if (!staged) stage_lists<BR, NT>(lv, lc, a, r0, split, KP, at, stored, tid);
                 staged = true;
             }
-            merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, LP, CP, capd, tid, NT);
+            merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, at, CP, capd, tid, NT);
             __syncthreads();
             // --- overflowed values retry against the raised tau (>=: an equal
             // value of this tile may still win on its lower column)
@@ -682,7 +775,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
         if constexpr (RING) {
             // the next tile's loads refill the ring: lists go back first
             if (staged) {
-                store_lists<BR, NT>(lv, lc, a, r0, split, KP, LP, tid);
+                store_lists<BR, NT>(lv, lc, a, r0, split, KP, at, tid);
                 stored = true;
                 staged = false;
                 __syncthreads();
@@ -691,10 +784,10 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     }
 #undef RC
     __syncthreads();
-    if constexpr (RING) stage_lists<BR, NT>(lv, lc, a, r0, split, KP, LP, stored, tid);
-    merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, LP, CP, capd, tid, NT);
+    if constexpr (RING) stage_lists<BR, NT>(lv, lc, a, r0, split, KP, at, stored, tid);
+    merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, at, CP, capd, tid, NT);
     __syncthreads();
-    store_lists<BR, NT>(lv, lc, a, r0, split, KP, LP, tid);
+    store_lists<BR, NT>(lv, lc, a, r0, split, KP, at, tid);
 }
 
 template <int ELEM, int WR, int BB>

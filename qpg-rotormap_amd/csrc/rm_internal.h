@@ -81,10 +81,13 @@ constexpr size_t topk_lds_bytes_for(int br, int kp) {
 }
 // The 256-query tile (k' <= 32) streams K through a ring of kRing stage
 // buffers filled by global -> LDS loads, and keeps each query's list in the
-// split partials (global memory), staged into the idle ring for merges:
+// split partials (global memory):
 // [ ring kRing x (kBI + br) x kRow ][ candidate slots br x (capd+1) x 8 ][ tau
 // br ][ slot counters br ][ 3 vote flags ].  The ring's load rows follow the
-// 64-byte swizzled row layout.
+// 64-byte swizzled row layout.  During the epilogue ring buffers 2 and 3 are
+// idle (the next tile's first stages may be landing in 0 and 1): a merge
+// stages the lists there, br x 32 values in buffer 2 and as many columns in
+// buffer 3, and the first tile's bound seed uses buffer 2.
 constexpr int kRing = 4;
 constexpr bool kRingOk = kSwz == 1;
 inline bool uses_ring(int kp) { return kRingOk && rows_per_tile(kp) == 256; }
@@ -98,8 +101,9 @@ constexpr int ring_cand_slots(int br) {
 constexpr size_t ring_lds_bytes(int br) {
     return ring_cand_offset(br) + (size_t)br * (ring_cand_slots(br) + 1) * 8 + tail_bytes(br);
 }
-// lists staged for a merge: br x (kp+1) values, then as many columns
-constexpr size_t ring_staged_list_bytes(int br, int kp) { return (size_t)br * (kp + 1) * 8; }
+constexpr size_t ring_buffer_bytes(int br) { return (size_t)(kBI + br) * kRow; }
+// one array (values or columns) of the lists staged for a merge
+constexpr size_t ring_staged_list_bytes(int br) { return (size_t)br * 32 * 4; }
 inline int topk_cand_slots(int kp) {
     return uses_ring(kp) ? ring_cand_slots(256) : cand_slots(rows_per_tile(kp), kp);
 }

@@ -12,15 +12,18 @@ int main() {
         const int br = rm::rows_per_tile(kp);
         if (rm::uses_ring(kp)) {
             // ring layout: the ring, then candidate slots, tau, counters, 3
-            // flags; a merge stages the lists (and the first tile its seed
-            // words) inside the ring
+            // flags; a merge stages the lists' values and columns in ring
+            // buffers 2 and 3 (32 entries per query), the first tile its seed
+            // words in buffer 2
             const size_t total = rm::topk_lds_bytes(kp);
             const int capd = rm::topk_cand_slots(kp);
             const size_t cand_end = rm::ring_cand_offset(br) + (size_t)br * (capd + 1) * 8;
             const size_t end = cand_end + (size_t)br * 8 + 3 * 4;
             if (capd < 1 || rm::ring_cand_offset(br) < rm::ring_bytes(br) || end > total ||
-                total > rm::kLdsMax || rm::ring_staged_list_bytes(br, kp) > rm::ring_bytes(br) ||
-                (size_t)br * 4 > rm::ring_bytes(br)) {
+                total > rm::kLdsMax || kp > 32 || rm::kRing < 4 ||
+                rm::ring_staged_list_bytes(br) > rm::ring_buffer_bytes(br) ||
+                (size_t)br * 4 > rm::ring_buffer_bytes(br) ||
+                rm::kRing * rm::ring_buffer_bytes(br) != rm::ring_bytes(br)) {
                 std::printf("ring kp=%d br=%d capd=%d: cands@%zu end %zu total %zu\n", kp, br, capd,
                             rm::ring_cand_offset(br), end, total);
                 ++bad;
