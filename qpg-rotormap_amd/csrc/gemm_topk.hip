@@ -711,10 +711,9 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                         if (acc[ai][bj][i] > tau) pend[bj] |= 1ull << (ai * 16 + i);
             }
         }
-        uint64_t any = 0;
-#pragma unroll
-        for (int bj = 0; bj < BB; ++bj) any |= pend[bj];
-        while (block_any(any != 0ull, flags, vote)) {
+        // One vote per tile: lanes append without a prior vote (a slot is
+        // reserved with an atomic, and merges only run after the vote below)
+        for (;;) {
             // --- append flagged values to the query's slots; a 16-value
             // group with no flag in any lane of the wave is skipped
 #pragma unroll
@@ -741,7 +740,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                     }
                 }
             }
-            any = 0;
+            uint64_t any = 0;
 #pragma unroll
             for (int bj = 0; bj < BB; ++bj) any |= pend[bj];
             // slots of every query still hold what they took: merge only when
@@ -755,7 +754,6 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
             __syncthreads();
             // --- overflowed values retry against the raised tau (>=: an equal
             // value of this tile may still win on its lower column)
-            any = 0;
 #pragma unroll
             for (int bj = 0; bj < BB; ++bj) {
                 if (!pend[bj]) continue;
@@ -769,7 +767,6 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                         if ((pend[bj] & bit) && !(acc[ai][bj][i] >= tau)) pend[bj] &= ~bit;
                     }
                 }
-                any |= pend[bj];
             }
         }
         if constexpr (RING) {
