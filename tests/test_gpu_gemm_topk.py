@@ -164,6 +164,35 @@ def test_fused_topk_fp16_bitexact(torch_gpu):
         np.testing.assert_array_equal(v, ev)
 
 
+@pytest.mark.parametrize("K", [64, 128, 192, 256, 320])
+def test_fused_topk_short_k_bitexact(torch_gpu, K):
+    """K of 1-5 64-byte stages (FP8) and 2-10 (FP16): stage counts that are
+    not a multiple of 4 take the fused kernel's per-tile K loop, the others
+    the loop that streams across tiles; both over 6 tiles with merges and
+    split counts 1 and 3."""
+    torch = torch_gpu
+    M, N, k = 300, 1500, 20
+    Q = rand_e4m3((M, K), 0.05, 50 + K)
+    B = rand_e4m3((N, K), 0.05, 60 + K)
+    ev, el = O.topk_rows(gemm_c(torch, Q, B), k)
+    eng = R.TopKEngine(B, k=k, mode="fp8")
+    for S in (1, 3):
+        v, l = fused(torch, eng, Q, splits=S)
+        np.testing.assert_array_equal(l, el, err_msg=f"fp8 K={K} splits={S}")
+        np.testing.assert_array_equal(v, ev, err_msg=f"fp8 K={K} splits={S}")
+    Q16 = (np.random.default_rng(70 + K).standard_normal((M, K)) * 0.05).astype(np.float16)
+    B16 = O.e4m3_table()[B].astype(np.float16)
+    C = torch.zeros(N, M, dtype=torch.float32, device="cuda").t()
+    R.fp8_gemm_hip(C, torch.from_numpy(Q16).cuda(), torch.from_numpy(B16).cuda(), N)
+    torch.cuda.synchronize()
+    ev, el = O.topk_rows(C.cpu().numpy(), k)
+    eng16 = R.TopKEngine(B, k=k, mode="fp16")
+    for S in (1, 3):
+        v, l = fused(torch, eng16, Q16, splits=S)
+        np.testing.assert_array_equal(l, el, err_msg=f"fp16 K={K} splits={S}")
+        np.testing.assert_array_equal(v, ev, err_msg=f"fp16 K={K} splits={S}")
+
+
 def test_fused_rescore_bitexact(torch_gpu):
     torch = torch_gpu
     M, N, K, k, kp = 200, 3000, 2048, 20, 32
