@@ -119,8 +119,9 @@ static int clamp_splits(const rm_index *idx, int S) {
     return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)S, cdiv(idx->N, kBI), (int64_t)256}));
 }
 
+// split partials (M x S x KP) and, after them, M shared bounds
 size_t part_bytes(const rm_index *idx, int64_t M, int KP, int S) {
-    return (size_t)M * clamp_splits(idx, S) * KP * 4;
+    return ((size_t)M * clamp_splits(idx, S) * KP + (size_t)M) * 4;
 }
 
 int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
@@ -729,7 +730,7 @@ int64_t rm_batch_bytes(int64_t M, int k, int rdim, int ktop, int kprime, int mod
     int64_t b = M * W * 4;                                  // packed words
     b += M * rdim * (mode == RM_MODE_FP16 ? 2 : 1);         // encoded queries
     if (mode == RM_MODE_FP8_RESCORE) b += M * rdim * 4;     // fp32 rescoring copy
-    b += M * 256LL * KP * 8;                                // split partials (<= 256 splits)
+    b += M * 256LL * KP * 8 + M * 8LL;                      // split partials (<= 256 splits), bounds
     b += M * (int64_t)ktop * 8;                             // vals + locs
     return b;
 }

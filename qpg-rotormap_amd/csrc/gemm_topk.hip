@@ -571,6 +571,30 @@ __device__ __forceinline__ void store_lists(const float *lv, const int32_t *lc, 
     }
 }
 
+// After a merge: a full list's k'-th value is a lower bound of its query's
+// global k'-th value (the list holds k' values at or above it); raise the
+// query's shared bound to it.  Run by the thread that merged the query.
+template <int BR, int NT>
+__device__ __forceinline__ void publish_bounds(const GemmArgs &a, const float *tauv, int64_t r0,
+                                               int tid) {
+    for (int q = tid; q < BR; q += NT)
+        if (r0 + q < a.M && tauv[q] > -INFINITY) atomicMax(&a.bound[r0 + q], order_key(tauv[q]));
+}
+
+// floor = the float just below the query's shared bound (values at or above
+// the bound pass a strict compare); -inf before any split has published.
+template <int BR, int NT>
+__device__ __forceinline__ void load_floor(const GemmArgs &a, float *floorv, int64_t r0, int tid) {
+    for (int q = tid; q < BR; q += NT) {
+        float f = -INFINITY;
+        if (r0 + q < a.M)
+            f = nextafterf(order_key_inv(__hip_atomic_load(&a.bound[r0 + q], __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT)),
+                           -INFINITY);
+        floorv[q] = f;
+    }
+}
+
 template <int ELEM, int WR, int BB>
 __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int split) {
     constexpr int BR = Tile<WR, BB>::BR;
@@ -597,6 +621,12 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     float *tauv = (float *)(cc + BR * CP);               // BR
     int *cnt = (int *)(tauv + BR);                       // BR
     int *flags = cnt + BR;                               // 3 vote slots
+    // ring layout with column splits: a per-query floor from the bound the
+    // splits of a query share (any split's k'-th value bounds the query's
+    // global k'-th value from below); values below it cannot reach the
+    // final top-k', so the filter keeps only values >= the bound
+    float *floorv = RING ? (float *)((uint8_t *)tauv + tail_bytes(BR)) : nullptr;  // BR
+    const bool share = RING && a.bound != nullptr;
 
     const int64_t r0 = rt * BR;
     const int64_t t0 = (int64_t)split * a.tiles / a.S, t1 = (int64_t)(split + 1) * a.tiles / a.S;
@@ -607,6 +637,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     // far (else they are empty); `staged` = they sit in LDS for this tile
     bool stored = false, staged = false;
     for (int i = tid; i < BR; i += NT) { tauv[i] = -INFINITY; cnt[i] = 0; }
+    if (share) load_floor<BR, NT>(a, floorv, r0, tid);
     if (tid < 3) flags[tid] = 0;
     int vote = 0;
     // (the first tile's barriers order this before any use)
@@ -696,7 +727,8 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
 #pragma unroll
         for (int bj = 0; bj < BB; ++bj) {
             pend[bj] = 0;
-            const float tau = tauv[ql[bj]];
+            float tau = tauv[ql[bj]];
+            if (share) tau = fmaxf(tau, floorv[ql[bj]]);  // floor: strictly above = at or above the bound
             float mx = -INFINITY;
 #pragma unroll
             for (int ai = 0; ai < 4; ++ai)
@@ -751,6 +783,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                 staged = true;
             }
             merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, at, CP, capd, tid, NT);
+            if (share) publish_bounds<BR, NT>(a, tauv, r0, tid);
             __syncthreads();
             // --- overflowed values retry against the raised tau (>=: an equal
             // value of this tile may still win on its lower column)
@@ -758,13 +791,15 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
             for (int bj = 0; bj < BB; ++bj) {
                 if (!pend[bj]) continue;
                 const float tau = tauv[ql[bj]];
+                const float fl = share ? floorv[ql[bj]] : -INFINITY;
 #pragma unroll
                 for (int ai = 0; ai < 4; ++ai) {
                     if (!((pend[bj] >> (ai * 16)) & 0xffffull)) continue;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
                         const uint64_t bit = 1ull << (ai * 16 + i);
-                        if ((pend[bj] & bit) && !(acc[ai][bj][i] >= tau)) pend[bj] &= ~bit;
+                        if ((pend[bj] & bit) && !(acc[ai][bj][i] >= tau && acc[ai][bj][i] > fl))
+                            pend[bj] &= ~bit;
                     }
                 }
             }
@@ -778,11 +813,15 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                 __syncthreads();
             }
         }
+        // every 4 tiles: the floor follows the shared bound (the next tile's
+        // barriers order the write before the next filter)
+        if (share && ((t - t0) & 3) == 3) load_floor<BR, NT>(a, floorv, r0, tid);
     }
 #undef RC
     __syncthreads();
     if constexpr (RING) stage_lists<BR, NT>(lv, lc, a, r0, split, KP, at, stored, tid);
     merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, at, CP, capd, tid, NT);
+    if (share) publish_bounds<BR, NT>(a, tauv, r0, tid);
     __syncthreads();
     store_lists<BR, NT>(lv, lc, a, r0, split, KP, at, tid);
 }
@@ -817,9 +856,26 @@ static int launch_common(const GemmArgs &a0, const void *fn, size_t lds, int nt,
 // 256 x 256 tiles, fed by the load ring; lists longer than 32 use 4 waves x
 // 128 x 64 (128 queries per tile) with register staging.
 
+// order key of -inf: the shared bounds' starting value
+constexpr int32_t kNegInfKey = (int32_t)(0xff800000u ^ 0x7fffffffu);
+
+__global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 int launch_gemm_topk(const GemmArgs &a0, int elem, hipStream_t st) {
     GemmArgs a = a0;
     a.capd = topk_cand_slots(a.KP);
+    a.bound = nullptr;
+    if (uses_ring(a.KP) && a.S > 1 && a.M > 0) {
+        // the bounds follow the partials (part_bytes holds M more entries);
+        // a fill kernel rather than a memset, so a captured graph replays it
+        a.bound = (int32_t *)(a.part_v + (size_t)a.M * a.S * a.KP);
+        fill_i32_kernel<<<dim3(stride_grid(a.M, 256)), dim3(256), 0, st>>>(a.bound, a.M, kNegInfKey);
+        RM_HIP(hipGetLastError());
+    }
     RM_CHECK(a.capd >= 1, RM_ERR_SHAPE, "no LDS left for candidate slots (k' = %d)", a.KP);
     const void *fn;
     int nt = 256;

@@ -83,7 +83,7 @@ constexpr size_t topk_lds_bytes_for(int br, int kp) {
 // buffers filled by global -> LDS loads, and keeps each query's list in the
 // split partials (global memory):
 // [ ring kRing x (kBI + br) x kRow ][ candidate slots br x (capd+1) x 8 ][ tau
-// br ][ slot counters br ][ 3 vote flags ].  The ring's load rows follow the
+// br ][ slot counters br ][ 3 vote flags ][ shared floor br ].  The ring's load rows follow the
 // 64-byte swizzled row layout.  During the epilogue ring buffers 2 and 3 are
 // idle (the next tile's first stages may be landing in 0 and 1): a merge
 // stages the lists there, br x 32 values in buffer 2 and as many columns in
@@ -93,13 +93,16 @@ constexpr bool kRingOk = kSwz == 1;
 inline bool uses_ring(int kp) { return kRingOk && rows_per_tile(kp) == 256; }
 constexpr size_t ring_bytes(int br) { return (size_t)kRing * (kBI + br) * kRow; }
 constexpr size_t ring_cand_offset(int br) { return ring_bytes(br); }
+// the ring layout's tail: tau, counters, flags, then the per-query floor
+// from the bound the column splits of a query share
+constexpr size_t ring_tail_bytes(int br) { return tail_bytes(br) + (size_t)br * 4; }
 constexpr int ring_cand_slots(int br) {
-    const size_t used = ring_cand_offset(br) + tail_bytes(br);
+    const size_t used = ring_cand_offset(br) + ring_tail_bytes(br);
     const long long avail = used >= kLdsMax ? 0 : (long long)((kLdsMax - used) / ((size_t)br * 8)) - 1;
     return avail > 64 ? 64 : (int)avail;
 }
 constexpr size_t ring_lds_bytes(int br) {
-    return ring_cand_offset(br) + (size_t)br * (ring_cand_slots(br) + 1) * 8 + tail_bytes(br);
+    return ring_cand_offset(br) + (size_t)br * (ring_cand_slots(br) + 1) * 8 + ring_tail_bytes(br);
 }
 constexpr size_t ring_buffer_bytes(int br) { return (size_t)(kBI + br) * kRow; }
 // one array (values or columns) of the lists staged for a merge
@@ -131,6 +134,10 @@ struct GemmArgs {
     int32_t *part_c;
     int KP;
     int capd;           // candidate slots per query (cand_slots)
+    // per-query lower bound of the k'-th score shared by a query's column
+    // splits (order keys, M entries after the partials; ring layout with
+    // S > 1), or null
+    int32_t *bound;
     // store-C epilogue
     void *C;
     int c_dtype;

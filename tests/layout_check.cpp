@@ -18,7 +18,7 @@ int main() {
             const size_t total = rm::topk_lds_bytes(kp);
             const int capd = rm::topk_cand_slots(kp);
             const size_t cand_end = rm::ring_cand_offset(br) + (size_t)br * (capd + 1) * 8;
-            const size_t end = cand_end + (size_t)br * 8 + 3 * 4;
+            const size_t end = cand_end + rm::tail_bytes(br) + (size_t)br * 4;  // + floor
             if (capd < 1 || rm::ring_cand_offset(br) < rm::ring_bytes(br) || end > total ||
                 total > rm::kLdsMax || kp > 32 || rm::kRing < 4 ||
                 rm::ring_staged_list_bytes(br) > rm::ring_buffer_bytes(br) ||

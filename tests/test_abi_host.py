@@ -65,12 +65,13 @@ def test_phase_table_bad_arguments():
 
 def test_batch_bytes_model():
     """Per-batch device bytes: packed words + encoded queries (+ fp32 copy
-    for rescoring) + split partials at the 256-split bound + outputs."""
+    for rescoring) + split partials at the 256-split bound and the splits'
+    shared bounds + outputs."""
     M, k, rdim = 8192, 20000, 2048
     W = (k + 15) // 16
     for mode, qb, extra, kp in [(L.RM_MODE_FP8, 1, 0, 20), (L.RM_MODE_FP8_RESCORE, 1, 4, 32),
                                 (L.RM_MODE_FP16, 2, 0, 20)]:
-        want = M * W * 4 + M * rdim * qb + M * rdim * extra + M * 256 * kp * 8 + M * 20 * 8
+        want = M * W * 4 + M * rdim * qb + M * rdim * extra + M * 256 * kp * 8 + M * 8 + M * 20 * 8
         assert L.lib().rm_batch_bytes(M, k, rdim, 20, 32, mode) == want
 
 
