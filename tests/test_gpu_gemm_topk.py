@@ -193,6 +193,40 @@ def test_fused_topk_short_k_bitexact(torch_gpu, K):
         np.testing.assert_array_equal(v, ev, err_msg=f"fp16 K={K} splits={S}")
 
 
+def test_fused_topk_random_plans_bitexact(torch_gpu):
+    """Seeded random shapes, K, k, column splits and query grouping in both
+    precisions: the fused top-k equals the oracle's total-order top-k of the
+    GPU's own C, value for value and column for column."""
+    torch = torch_gpu
+    rng = np.random.default_rng(2024)
+    for case in range(12):
+        M = int(rng.integers(1, 700))
+        N = int(rng.integers(1, 6000))
+        K = 64 * int(rng.integers(1, 17))
+        k = int(rng.choice([1, 5, 20, 32]))
+        fp16 = bool(case % 3 == 2)
+        tiles = -(-N // 256)
+        S = int(rng.integers(1, min(tiles, 9) + 1))
+        qg = int(rng.choice([0, 1, 2, 3]))
+        B = rand_e4m3((N, K), 0.05, 100 + case)
+        if fp16:
+            Q = (rng.standard_normal((M, K)) * 0.05).astype(np.float16)
+            C = torch.zeros(N, M, dtype=torch.float32, device="cuda").t()
+            B16 = O.e4m3_table()[B].astype(np.float16)
+            R.fp8_gemm_hip(C, torch.from_numpy(Q).cuda(), torch.from_numpy(B16).cuda(), N)
+            torch.cuda.synchronize()
+            C = C.cpu().numpy()
+        else:
+            Q = rand_e4m3((M, K), 0.05, 200 + case)
+            C = gemm_c(torch, Q, B)
+        ev, el = O.topk_rows(C, k)
+        eng = R.TopKEngine(B, k=k, mode="fp16" if fp16 else "fp8")
+        v, l = fused(torch, eng, Q, splits=S, qg=qg)
+        msg = f"case {case}: M={M} N={N} K={K} k={k} S={S} qg={qg} fp16={fp16}"
+        np.testing.assert_array_equal(l, el, err_msg=msg)
+        np.testing.assert_array_equal(v, ev, err_msg=msg)
+
+
 def test_fused_rescore_bitexact(torch_gpu):
     torch = torch_gpu
     M, N, K, k, kp = 200, 3000, 2048, 20, 32
