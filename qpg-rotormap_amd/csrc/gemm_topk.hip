@@ -266,9 +266,14 @@ __device__ __forceinline__ RingSrc ring_src(const TileSrc &ts, int tid) {
     return d;
 }
 
+// loads per wave per stage: 512 rows over 8 waves, 16 rows per load
+constexpr int kRingLoads = Tile<4, 2>::ROWS / 8 / 16;
+static_assert(kRingLoads == 4, "the ring's vmcnt waits count 4 loads per stage");
+static_assert((kRing - 2) * kRingLoads == 8, "ring_stage waits with vmcnt(8)");
+
 __device__ __forceinline__ void ring_load(const RingSrc &d, uint8_t *sb, int kb) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < kRingLoads; ++j)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
             d.r, (__attribute__((address_space(3))) void *)(sb + d.lds_off + j * 16 * kRow), 16,
             d.voff, kb + j * d.step16, 0, 0);
@@ -285,8 +290,7 @@ template <int ELEM, bool FIRST>
 __device__ __forceinline__ void ring_stage(v16f (&acc)[4][2], const RingSrc &d,
                                            uint8_t *__restrict__ bw, const uint8_t *__restrict__ br,
                                            int s, int nk, int wi, int wr, int lane) {
-    static_assert(kRing == 4, "vmcnt below counts kRing - 2 stages of 4 loads");
-    asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");  // (kRing-2)*kRingLoads
     Frags<2> f;
     ld_frags_b<ELEM, 2>(f, br, wr, lane);
     ld_frags_a<ELEM, 2, 0, 2>(f, br, wi, lane);
