@@ -495,6 +495,24 @@ __global__ __launch_bounds__(128 * WR, 1) void gemm_store_kernel(GemmArgs a) {
     }
 }
 
+// An opaque copy of v.  Addresses computed from it stay in the rarely taken
+// paths that use them, instead of being hoisted out of the tile loop and
+// held in registers (or spilled) across the K loop.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// A wave-uniform opaque copy of v (an SGPR).  The ring base is smem plus an
+// opaque zero: with the constant smem, interprocedural constant propagation
+// would substitute it into a stage's restrict buffer parameter wherever every
+// call passes the same buffer, and the fragment reads would lose the alias
+// scope that keeps the compiler from waiting for every LDS load in flight.
+__device__ __forceinline__ int opaque_uniform(int v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
 // LDS index of entry j of query q's list: rows of LP = k' + 1 entries, or
 // (ROT, lists staged in the ring, k' <= 32) rows of 32 entries rotated by q,
 // so that a warp-wide step of the one-thread-per-query merge (same j,
@@ -514,7 +532,7 @@ __device__ __forceinline__ void merge_slots(float *lv, int32_t *lc, const float 
                                             const int32_t *cc, float *tauv, int *cnt, int BR,
                                             int KP, ListAt<ROT> at, int CP, int capd, int tid,
                                             int nt) {
-    for (int q = tid; q < BR; q += nt) {
+    for (int q = opaque(tid); q < BR; q += nt) {
         const int nc = min(cnt[q], capd);
         if (nc == 0) continue;
         for (int j = 0; j < nc; ++j) {
@@ -544,7 +562,7 @@ template <int BR, int NT, bool ROT>
 __device__ __forceinline__ void stage_lists(float *lv, int32_t *lc, const GemmArgs &a, int64_t r0,
                                             int split, int KP, ListAt<ROT> at, bool stored,
                                             int tid) {
-    for (int q = tid; q < BR; q += NT) {
+    for (int q = opaque(tid); q < BR; q += NT) {
         if (stored && r0 + q < a.M) {
             const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
             for (int j = 0; j < KP; ++j) {
@@ -565,7 +583,7 @@ template <int BR, int NT, bool ROT>
 __device__ __forceinline__ void store_lists(const float *lv, const int32_t *lc, const GemmArgs &a,
                                             int64_t r0, int split, int KP, ListAt<ROT> at,
                                             int tid) {
-    for (int q = tid; q < BR; q += NT) {
+    for (int q = opaque(tid); q < BR; q += NT) {
         if (r0 + q >= a.M) continue;
         const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
         for (int j = 0; j < KP; ++j) {
@@ -581,7 +599,7 @@ __device__ __forceinline__ void store_lists(const float *lv, const int32_t *lc, 
 template <int BR, int NT>
 __device__ __forceinline__ void publish_bounds(const GemmArgs &a, const float *tauv, int64_t r0,
                                                int tid) {
-    for (int q = tid; q < BR; q += NT)
+    for (int q = opaque(tid); q < BR; q += NT)
         if (r0 + q < a.M && tauv[q] > -INFINITY) atomicMax(&a.bound[r0 + q], order_key(tauv[q]));
 }
 
@@ -589,7 +607,7 @@ __device__ __forceinline__ void publish_bounds(const GemmArgs &a, const float *t
 // the bound pass a strict compare); -inf before any split has published.
 template <int BR, int NT>
 __device__ __forceinline__ void load_floor(const GemmArgs &a, float *floorv, int64_t r0, int tid) {
-    for (int q = tid; q < BR; q += NT) {
+    for (int q = opaque(tid); q < BR; q += NT) {
         float f = -INFINITY;
         if (r0 + q < a.M)
             f = nextafterf(order_key_inv(__hip_atomic_load(&a.bound[r0 + q], __ATOMIC_RELAXED,
@@ -657,14 +675,15 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     const int rowbase = wi * 128 + 4 * (lane >> 5);
 #define RC(ai, i) ((ai) * 32 + 8 * ((i) >> 2) + ((i) & 3))
     v16f acc[4][BB];
+    uint8_t *const ring = smem + (RING ? opaque_uniform(0) : 0);
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
         if constexpr (RING) {
             if ((a.Kb / kBK) % 4 == 0)
-                tile_product_ring_stream<ELEM>(acc, a, i0, r0, smem, tid, wi, wr, lane, t == t0,
+                tile_product_ring_stream<ELEM>(acc, a, i0, r0, ring, tid, wi, wr, lane, t == t0,
                                                t + 1 < t1);
             else
-                tile_product_ring<ELEM>(acc, a, i0, r0, smem, tid, wi, wr, lane);
+                tile_product_ring<ELEM>(acc, a, i0, r0, ring, tid, wi, wr, lane);
         } else
             tile_product<ELEM, WR, BB>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
         // columns of this tile valid for this lane: RC(ai, i) < lim
