@@ -23,8 +23,10 @@ Workloads (synthetic; one step = one batch of reads resident in HBM):
                     2-bit packed on the GPU; checked equal to the packed reads
   serving           Poisson request traces at 50 / 200 / 1000 req/s (human: also
                     10 req/s, below saturation; reads per request log-uniform
-                    on [1, 100000]) through the continuous-batching server:
-                    p50 / p99 / p999 latency, offered vs achieved reads/s
+                    on [1, 100000]) through the continuous-batching server
+                    (queued rows fill a launch fewest-remaining-first;
+                    --serve-policy fifo for arrival order): p50 / p99 / p999
+                    latency, offered vs achieved reads/s
 
 python bench.py [--workload human|100mbp] [--steps K] [--warmup W] [--gpus 1]
                 [--reads M] [--no-cpu] [--no-serve] [--no-uneven] [--no-fasta]
@@ -128,6 +130,8 @@ def main():
                     help="requests per rate (100mbp: 200,400,600; human: 40 each)")
     ap.add_argument("--serve-rows", type=int, default=16384, help="rows per serving launch")
     ap.add_argument("--serve-streams", type=int, default=2, help="serving launch slots")
+    ap.add_argument("--serve-policy", choices=["fifo", "srpt"], default="srpt",
+                    help="order in which queued rows fill a launch")
     ap.add_argument("--uneven-batches", type=int, default=48,
                     help="first batches of the 4096-size uneven sequence (human)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -393,10 +397,11 @@ def main():
         rates = [float(x) for x in serve_rates.split(",")]
         nreq = [int(x) for x in serve_req.split(",")]
         serving = {"mode": "fp8_rescore", "rows_per_launch": args.serve_rows,
-                   "launch_slots": args.serve_streams,
+                   "launch_slots": args.serve_streams, "policy": args.serve_policy,
                    "request_reads": "log-uniform on [1, 100000]", "pool_reads": POOL, "rates": []}
         with R.Server(eng8.index, re, k=20, kprime=32, mode="fp8_rescore",
-                      max_rows=args.serve_rows, n_streams=args.serve_streams) as srv:
+                      max_rows=args.serve_rows, n_streams=args.serve_streams,
+                      policy=args.serve_policy) as srv:
             for i, (rate, nr) in enumerate(zip(rates, nreq)):
                 t, n = R.poisson_trace(nr, rate, seed=100 + i)
                 res = R.run_trace(srv, pinned, t, n, seed=200 + i)

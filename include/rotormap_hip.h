@@ -234,6 +234,16 @@ int rm_server_wait(rm_server *srv, int64_t req_id, int timeout_ms, float *host_v
 /* Cancel a request that has not been admitted yet. */
 int rm_server_cancel(rm_server *srv, int64_t req_id);
 int rm_server_stats(rm_server *srv, int64_t *launches, int64_t *rows_done, int64_t *max_inflight);
+/* Order in which the dispatcher fills a launch with queued rows.
+ * RM_SCHED_FIFO: arrival order (a request's rows before any later request's).
+ * RM_SCHED_SRPT: fewest remaining rows first, ties in arrival order, so a
+ * small request that arrives behind a large one rides the next launch.  Every
+ * launch is still filled to the row capacity; results do not depend on the
+ * order.  Takes effect from the next launch.  No counterpart in the reference
+ * (it has no serving loop). */
+#define RM_SCHED_FIFO 0
+#define RM_SCHED_SRPT 1
+int rm_server_set_policy(rm_server *srv, int policy);
 /* Seconds on the server's monotonic clock (the clock of the t_* stamps). */
 double rm_server_now(rm_server *srv);
 /* Stop the loop: requests not yet launched end as cancelled (or failed when

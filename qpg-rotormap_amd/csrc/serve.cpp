@@ -86,6 +86,7 @@ struct rm_server {
     std::deque<Slot *> inflight;
     std::vector<Slot *> free_slots;
     int64_t next_id = 0;
+    int policy = RM_SCHED_FIFO;
     bool stop = false;
     int64_t launches = 0, rows_done = 0, max_inflight = 0;
     clk::time_point t0 = clk::now();
@@ -205,8 +206,15 @@ void dispatcher(rm_server *sv) {
         sl->pieces.clear();
         int64_t rows = 0;
         const double t = sv->now();
-        while (!sv->queue.empty() && rows < sv->max_rows) {
-            Req *r = sv->queue.front();
+        // the queue stays in arrival order; SRPT visits it by remaining rows
+        std::vector<Req *> order(sv->queue.begin(), sv->queue.end());
+        if (sv->policy == RM_SCHED_SRPT)
+            std::stable_sort(order.begin(), order.end(), [](const Req *x, const Req *y) {
+                return x->n - x->next_row < y->n - y->next_row;
+            });
+        bool emptied = false;
+        for (Req *r : order) {
+            if (rows == sv->max_rows) break;
             const int64_t take = std::min(r->n - r->next_row, sv->max_rows - rows);
             sl->pieces.push_back({r, r->next_row, r->next_row + take, rows});
             ++r->pending;
@@ -216,8 +224,12 @@ void dispatcher(rm_server *sv) {
             }
             r->next_row += take;
             rows += take;
-            if (r->next_row == r->n) sv->queue.pop_front();
+            emptied |= r->next_row == r->n;
         }
+        if (emptied)
+            sv->queue.erase(std::remove_if(sv->queue.begin(), sv->queue.end(),
+                                           [](const Req *r) { return r->next_row == r->n; }),
+                            sv->queue.end());
         sl->rows = rows;
         lk.unlock();
         const int rc = launch_slot(sv, sl);
@@ -429,6 +441,15 @@ int rm_server_cancel(rm_server *srv, int64_t req_id) {
     r->state = Q_CANCELLED;
     r->t_fin = srv->now();
     srv->cv.notify_all();
+    return RM_OK;
+}
+
+int rm_server_set_policy(rm_server *srv, int policy) {
+    RM_CHECK(srv, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(policy == RM_SCHED_FIFO || policy == RM_SCHED_SRPT, RM_ERR_ARG,
+             "policy must be RM_SCHED_FIFO or RM_SCHED_SRPT");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->policy = policy;
     return RM_OK;
 }
 

@@ -14,6 +14,7 @@ RM_OK, RM_ERR_ARG, RM_ERR_SHAPE, RM_ERR_NOMEM, RM_ERR_HIP, RM_ERR_STATE = range(
 RM_F32, RM_F16, RM_E4M3 = 0, 1, 2
 RM_MODE_FP8, RM_MODE_FP8_RESCORE, RM_MODE_FP16 = 0, 1, 2
 RM_INDEX_HOST, RM_INDEX_FP16 = 1, 2
+RM_SCHED_FIFO, RM_SCHED_SRPT = 0, 1
 
 _CODE_NAMES = {1: "bad argument", 2: "shape", 3: "out of memory", 4: "HIP error", 5: "state"}
 
@@ -75,6 +76,7 @@ SIGNATURES = {
     "rm_server_wait": (i32, [vp, i64, i32, vp, vp, vp, vp, vp]),
     "rm_server_cancel": (i32, [vp, i64]),
     "rm_server_stats": (i32, [vp, vp, vp, vp]),
+    "rm_server_set_policy": (i32, [vp, i32]),
     "rm_server_destroy": (i32, [vp]),
     "rm_fasta_reads": (i32, [vp, vp, i64, i32, vp, i64, vp, vp, C.POINTER(i64), vp]),
     "rm_fasta_reads_host": (i32, [vp, vp, i64, i32, vp, i64, vp, vp, C.POINTER(i64)]),

@@ -1,10 +1,11 @@
 """Serving on one GPU: continuous batching over a resident index, and a
 Poisson load generator that reports p50 / p99 / p999 request latency.
 
-The loop itself is native (rm_server_* in librotormap_hip.so): a FIFO of
-requests, a dispatcher that packs rows from the FIFO head into the next free
-launch slot (a request larger than a slot is spread over launches, a launch
-carries several requests), slots with their own streams and buffers
+The loop itself is native (rm_server_* in librotormap_hip.so): a queue of
+requests, a dispatcher that packs queued rows into the next free launch slot
+in arrival order ("fifo") or fewest remaining rows first ("srpt"; a request
+larger than a slot is spread over launches, a launch carries several
+requests), slots with their own streams and buffers
 allocated once, and a completer that scatters finished rows back to their
 requests.  The reference has no serving loop (its "server" mode is a file
 watcher, RotorMap/legacy/src/RotorMap.jl:233); the per-launch work is
@@ -43,10 +44,11 @@ class ServedRequest:
 class Server:
     """rm_server over `index` for encoder `re`.  max_rows: rows per launch;
     n_streams: launch slots in flight; hbm_budget: byte cap on the slots'
-    buffers (0 = the free HBM less 96 MiB)."""
+    buffers (0 = the free HBM less 96 MiB); policy: "fifo" or "srpt", the
+    order in which queued rows fill a launch (set_policy changes it)."""
 
     def __init__(self, index, re: RopeEncoder, k=20, kprime=32, mode="fp8_rescore",
-                 max_rows=65536, n_streams=2, hbm_budget=0, device=0):
+                 max_rows=65536, n_streams=2, hbm_budget=0, device=0, policy="fifo"):
         if not isinstance(index, ResidentIndex):
             index = ResidentIndex(index, fp16=(mode == "fp16"), device=device)
         self.index, self.re, self.k = index, re, int(k)
@@ -58,6 +60,15 @@ class Server:
         self.handle = h
         self._refs = {}
         self._mu = threading.Lock()
+        self.set_policy(policy)
+
+    def set_policy(self, policy):
+        """"fifo" (arrival order) or "srpt" (fewest remaining rows first)."""
+        codes = {"fifo": L.RM_SCHED_FIFO, "srpt": L.RM_SCHED_SRPT}
+        if policy not in codes:
+            raise ValueError(f"policy must be one of {sorted(codes)}, got {policy!r}")
+        L.check(L.lib().rm_server_set_policy(self.handle, codes[policy]))
+        self.policy = policy
 
     def submit(self, words, copy=True):
         """Enqueue n fragments: words (n, W) uint32 (or flat n*W).  copy=False

@@ -39,7 +39,8 @@ def test_version_and_status_codes():
     assert b"gfx950" in L.lib().rm_version()
     text = open(HEADER).read()
     for name, val in [("RM_OK", 0), ("RM_ERR_ARG", 1), ("RM_ERR_SHAPE", 2), ("RM_ERR_NOMEM", 3),
-                      ("RM_ERR_HIP", 4), ("RM_ERR_STATE", 5)]:
+                      ("RM_ERR_HIP", 4), ("RM_ERR_STATE", 5),
+                      ("RM_SCHED_FIFO", L.RM_SCHED_FIFO), ("RM_SCHED_SRPT", L.RM_SCHED_SRPT)]:
         assert re.search(rf"#define\s+{name}\s+{val}\b", text), name
 
 
@@ -53,6 +54,7 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.rm_topk_batch(None, None, None, 2048, None, 2048, 1, 20, 32, 1, None, None, 1,
                              None) == L.RM_ERR_ARG
     assert lib.rm_server_wait(None, 0, 0, None, None, None, None, None) == L.RM_ERR_ARG
+    assert lib.rm_server_set_policy(None, L.RM_SCHED_SRPT) == L.RM_ERR_ARG
     assert lib.rm_server_destroy(None) == L.RM_OK
     assert lib.rm_index_cols(None) == 0
     assert lib.rm_device_free_bytes(None) == -1

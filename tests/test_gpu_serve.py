@@ -66,6 +66,47 @@ def test_server_alone_equals_mixed(torch_gpu, setup):
     np.testing.assert_array_equal(a.locs, b.locs)
 
 
+@pytest.mark.parametrize("policy", ["srpt", "fifo"])
+def test_server_policy_order(torch_gpu, setup, policy):
+    """One launch slot of 256 rows; a 1024-row request, then a 10-row one.
+    Shortest-remaining-first puts the small request in the first or second
+    launch, so it finishes strictly before the large one (which needs four).
+    In arrival order its rows follow all of the large request's.  Both
+    orders return the one-shot rows."""
+    ref, re, idx, reads = setup
+    v0, l0 = oneshot(torch_gpu, idx, re, reads.words[:1034], "fp8_rescore")
+    with R.Server(idx.embeds8, re, max_rows=256, n_streams=1, policy=policy) as srv:
+        big = srv.submit(reads.words[:1024])
+        small = srv.submit(reads.words[1024:1034])
+        b, s = srv.wait(*big), srv.wait(*small)
+        st = srv.stats()
+    np.testing.assert_array_equal(b.vals, v0[:1024])
+    np.testing.assert_array_equal(b.locs, l0[:1024])
+    np.testing.assert_array_equal(s.vals, v0[1024:])
+    np.testing.assert_array_equal(s.locs, l0[1024:])
+    if policy == "srpt":
+        assert s.t_finish < b.t_finish
+    else:
+        assert s.t_finish >= b.t_finish
+    assert st["rows_done"] == 1034 and st["launches"] == 5
+
+
+def test_server_srpt_matches_oneshot(torch_gpu, setup):
+    ref, re, idx, reads = setup
+    v0, l0 = oneshot(torch_gpu, idx, re, reads.words, "fp8_rescore")
+    sizes = [1, 0, 650, 1400, 3, 700, 246]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    with R.Server(idx.embeds8, re, max_rows=700, n_streams=2, policy="srpt") as srv:
+        ids = [srv.submit(reads.words[offs[i]:offs[i + 1]]) for i in range(len(sizes))]
+        out = [srv.wait(rid, n) for rid, n in ids]
+        with pytest.raises(ValueError):
+            srv.set_policy("lifo")
+    for i, r in enumerate(out):
+        np.testing.assert_array_equal(r.vals, v0[offs[i]:offs[i + 1]])
+        np.testing.assert_array_equal(r.locs, l0[offs[i]:offs[i + 1]])
+        assert r.t_submit <= r.t_admit <= r.t_finish
+
+
 def test_server_admission_and_errors(torch_gpu, setup):
     ref, re, idx, reads = setup
     B = R.ResidentIndex(idx.embeds8)
