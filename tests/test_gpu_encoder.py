@@ -102,6 +102,25 @@ def test_encoder_k20000_both_index_configs(torch_gpu):
             assert abs(e - 1.0) < 1e-5  # unit energy (RotorMap/encode/kernel.jl:378-383)
 
 
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+@pytest.mark.parametrize("normalize", [0, 1, 2, 3])
+def test_encoder_unit_energy_every_row(torch_gpu, dtype, normalize):
+    """The reference's streamed-batch check (RotorMap/encode/stream.jl:274-287):
+    every row has unit energy over its [Re; Im] halves, the whole row for
+    normalize 0, else the largest copy (stride m within each half), to rtol
+    1e-3 for fp32 storage and 1e-2 for fp16."""
+    k, (s, m, c) = 2001, (8, 4, 4)
+    re = R.RopeEncoder(k=k, s=s, m=m, c=c)
+    words = random_words(96, k, seed=31 + normalize)
+    out, _, _ = gpu_encode(torch_gpu, words, re, normalize, dtype)
+    x = out.astype(np.float64)
+    if normalize == 0:
+        e = np.sum(x * x, axis=1)
+    else:
+        e = np.max(np.stack([np.sum(x[:, i::m] ** 2, axis=1) for i in range(m)]), axis=0)
+    np.testing.assert_allclose(e, 1.0, rtol=1e-3 if dtype == "f32" else 1e-2)
+
+
 def test_encoder_deterministic_across_batches(torch_gpu):
     k, (s, m, c) = 2001, (8, 4, 4)
     re = R.RopeEncoder(k=k, s=s, m=m, c=c)
@@ -140,6 +159,9 @@ def test_revcomp_and_extract(torch_gpu):
 
 
 def test_quantize_e4m3_exhaustive(torch_gpu):
+    """Device e4m3 bits equal the host conversion, as the reference's
+    _check_quantize_path asserts (RotorMap/search/engine_fp8.jl:600-617),
+    here over every binary16 value and beyond the saturation point."""
     torch = torch_gpu
     # every binary16 value (as fp32) plus random fp32 values incl. saturation
     h = np.arange(65536, dtype=np.uint16).view(np.float16).astype(np.float32)
