@@ -236,8 +236,10 @@ int rm_server_cancel(rm_server *srv, int64_t req_id);
 int rm_server_stats(rm_server *srv, int64_t *launches, int64_t *rows_done, int64_t *max_inflight);
 /* Order in which the dispatcher fills a launch with queued rows.
  * RM_SCHED_FIFO: arrival order (a request's rows before any later request's).
- * RM_SCHED_SRPT: fewest remaining rows first, ties in arrival order, so a
- * small request that arrives behind a large one rides the next launch.  Every
+ * RM_SCHED_SRPT: the oldest queued request takes up to a quarter of the
+ * launch (it keeps progressing however many small requests arrive), then
+ * fewest remaining rows first, ties in arrival order, so a small request that
+ * arrives behind a large one rides the next launch.  Every
  * launch is still filled to the row capacity; results do not depend on the
  * order.  Takes effect from the next launch.  No counterpart in the reference
  * (it has no serving loop). */

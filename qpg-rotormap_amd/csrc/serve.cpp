@@ -206,16 +206,10 @@ void dispatcher(rm_server *sv) {
         sl->pieces.clear();
         int64_t rows = 0;
         const double t = sv->now();
-        // the queue stays in arrival order; SRPT visits it by remaining rows
-        std::vector<Req *> order(sv->queue.begin(), sv->queue.end());
-        if (sv->policy == RM_SCHED_SRPT)
-            std::stable_sort(order.begin(), order.end(), [](const Req *x, const Req *y) {
-                return x->n - x->next_row < y->n - y->next_row;
-            });
         bool emptied = false;
-        for (Req *r : order) {
-            if (rows == sv->max_rows) break;
-            const int64_t take = std::min(r->n - r->next_row, sv->max_rows - rows);
+        auto take_rows = [&](Req *r, int64_t want) {
+            const int64_t take = std::min({r->n - r->next_row, sv->max_rows - rows, want});
+            if (take <= 0) return;
             sl->pieces.push_back({r, r->next_row, r->next_row + take, rows});
             ++r->pending;
             if (r->state == Q_QUEUED) {
@@ -225,6 +219,20 @@ void dispatcher(rm_server *sv) {
             r->next_row += take;
             rows += take;
             emptied |= r->next_row == r->n;
+        };
+        // the queue stays in arrival order; SRPT visits it by remaining rows,
+        // after the oldest request has taken up to a quarter of the launch
+        // (so a large request keeps progressing while small ones keep coming)
+        std::vector<Req *> order(sv->queue.begin(), sv->queue.end());
+        if (sv->policy == RM_SCHED_SRPT) {
+            take_rows(order.front(), std::max<int64_t>(1, sv->max_rows / 4));
+            std::stable_sort(order.begin(), order.end(), [](const Req *x, const Req *y) {
+                return x->n - x->next_row < y->n - y->next_row;
+            });
+        }
+        for (Req *r : order) {
+            if (rows == sv->max_rows) break;
+            take_rows(r, sv->max_rows);
         }
         if (emptied)
             sv->queue.erase(std::remove_if(sv->queue.begin(), sv->queue.end(),
