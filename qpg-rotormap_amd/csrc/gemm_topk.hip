@@ -553,28 +553,87 @@ __device__ __forceinline__ void merge_slots(float *lv, int32_t *lc, const float 
     }
 }
 
+// The value of lane l - 1 (a whole-wave DPP shift right by one lane; lane 0
+// keeps its own value).
+__device__ __forceinline__ int lane_up(int x) {
+    return __builtin_amdgcn_update_dpp(x, x, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+}
+
+// The same merge as merge_slots for lists staged as rotated 32-entry rows
+// (k' <= 32), with 32 lanes per query: lane j holds entry j in registers.  A
+// candidate's place p is the number of the half-wave's entries that beat it
+// (a ballot; the 32 lanes stay sorted, so p >= k' only for a candidate that
+// drops out), and lanes j > p take lane j - 1's entry (the shift into lane 32
+// from lane 31 is never taken: p >= 0).  Keys (value, column) are distinct,
+// so the lists equal those of one-by-one insertion.  All NT threads take part,
+// two queries per wave.
+template <int BR, int NT>
+__device__ __forceinline__ void merge_slots_lanes(float *lv, int32_t *lc, const float *cv,
+                                                  const int32_t *cc, float *tauv, int *cnt,
+                                                  int KP, ListAt<true> at, int CP, int capd,
+                                                  int tid) {
+    const int t = opaque(tid), lane = t & 63, j = lane & 31, half = lane >> 5;
+    for (int q0 = (t >> 6) * 2; q0 < BR; q0 += NT / 32) {
+        const int q = q0 + half;
+        const int nc = min(cnt[q], capd);
+        const int ncmax = max(__builtin_amdgcn_readlane(nc, 0), __builtin_amdgcn_readlane(nc, 32));
+        if (ncmax == 0) continue;
+        float v = -INFINITY;
+        int32_t c = INT_MAX;
+        if (j < KP) {
+            v = lv[at(q, j)];
+            c = lc[at(q, j)];
+        }
+        for (int i = 0; i < ncmax; ++i) {
+            const int ci = q * CP + min(i, max(nc - 1, 0));
+            const float x = cv[ci];
+            const int32_t xc = cc[ci];
+            const uint64_t m = __ballot(beats(v, c, x, xc));
+            const int p = __popc((uint32_t)(m >> (32 * half)));
+            const float vs = __int_as_float(lane_up(__float_as_int(v)));
+            const int32_t cs = lane_up(c);
+            if (i < nc && p < KP) {
+                if (j == p) {
+                    v = x;
+                    c = xc;
+                } else if (j > p) {
+                    v = vs;
+                    c = cs;
+                }
+            }
+        }
+        if (j < KP) {
+            lv[at(q, j)] = v;
+            lc[at(q, j)] = c;
+        }
+        if (j == KP - 1) tauv[q] = v;
+        if (j == 0) cnt[q] = 0;
+    }
+}
+
 // Ring layout: a (query, split) list lives in the split partials between
 // merges.  For a merge it is staged into the two ring buffers that are idle
 // during the epilogue: from the partials once a merge has stored it, else
-// empty.  One thread per query, the same thread that stores it back, so no
-// other thread reads a list between its store and its next load.
+// empty.  All threads take part, 32 lanes per query (lane = entry), so each
+// wave moves two contiguous list rows per access; the barriers around a merge
+// order a store before any later load of the same list.
 template <int BR, int NT, bool ROT>
 __device__ __forceinline__ void stage_lists(float *lv, int32_t *lc, const GemmArgs &a, int64_t r0,
                                             int split, int KP, ListAt<ROT> at, bool stored,
                                             int tid) {
-    for (int q = opaque(tid); q < BR; q += NT) {
+    static_assert(ROT, "staged lists use the rotated 32-entry rows");
+    const int t = opaque(tid), j = t & 31;
+    for (int q = t >> 5; q < BR; q += NT / 32) {
+        if (j >= KP) continue;
+        float v = -INFINITY;
+        int32_t c = INT_MAX;
         if (stored && r0 + q < a.M) {
-            const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
-            for (int j = 0; j < KP; ++j) {
-                lv[at(q, j)] = a.part_v[base + j];
-                lc[at(q, j)] = a.part_c[base + j];
-            }
-        } else {
-            for (int j = 0; j < KP; ++j) {
-                lv[at(q, j)] = -INFINITY;
-                lc[at(q, j)] = INT_MAX;
-            }
+            const int64_t e = ((r0 + q) * a.S + split) * (int64_t)KP + j;
+            v = a.part_v[e];
+            c = a.part_c[e];
         }
+        lv[at(q, j)] = v;
+        lc[at(q, j)] = c;
     }
     __syncthreads();
 }
@@ -583,10 +642,11 @@ template <int BR, int NT, bool ROT>
 __device__ __forceinline__ void store_lists(const float *lv, const int32_t *lc, const GemmArgs &a,
                                             int64_t r0, int split, int KP, ListAt<ROT> at,
                                             int tid) {
-    for (int q = opaque(tid); q < BR; q += NT) {
+    const int t = opaque(tid);
+    for (int q = t >> 5; q < BR; q += NT / 32) {
         if (r0 + q >= a.M) continue;
         const int64_t base = ((r0 + q) * a.S + split) * (int64_t)KP;
-        for (int j = 0; j < KP; ++j) {
+        for (int j = t & 31; j < KP; j += 32) {
             a.part_v[base + j] = lv[at(q, j)];
             a.part_c[base + j] = lc[at(q, j)];
         }
@@ -805,9 +865,12 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                 if (!staged) stage_lists<BR, NT>(lv, lc, a, r0, split, KP, at, stored, tid);
                 staged = true;
             }
-            merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, at, CP, capd, tid, NT);
-            if (share) publish_bounds<BR, NT>(a, tauv, r0, tid);
+            if constexpr (RING)
+                merge_slots_lanes<BR, NT>(lv, lc, cv, cc, tauv, cnt, KP, at, CP, capd, tid);
+            else
+                merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, at, CP, capd, tid, NT);
             __syncthreads();
+            if (share) publish_bounds<BR, NT>(a, tauv, r0, tid);
             // --- overflowed values retry against the raised tau (>=: an equal
             // value of this tile may still win on its lower column)
 #pragma unroll
@@ -842,10 +905,14 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     }
 #undef RC
     __syncthreads();
-    if constexpr (RING) stage_lists<BR, NT>(lv, lc, a, r0, split, KP, at, stored, tid);
-    merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, at, CP, capd, tid, NT);
-    if (share) publish_bounds<BR, NT>(a, tauv, r0, tid);
+    if constexpr (RING) {
+        stage_lists<BR, NT>(lv, lc, a, r0, split, KP, at, stored, tid);
+        merge_slots_lanes<BR, NT>(lv, lc, cv, cc, tauv, cnt, KP, at, CP, capd, tid);
+    } else {
+        merge_slots(lv, lc, cv, cc, tauv, cnt, BR, KP, at, CP, capd, tid, NT);
+    }
     __syncthreads();
+    if (share) publish_bounds<BR, NT>(a, tauv, r0, tid);
     store_lists<BR, NT>(lv, lc, a, r0, split, KP, at, tid);
 }
 
