@@ -602,12 +602,16 @@ __device__ __forceinline__ void merge_slots_lanes(float *lv, int32_t *lc, const 
                 }
             }
         }
-        if (j < KP) {
-            lv[at(q, j)] = v;
-            lc[at(q, j)] = c;
+        // a query without candidates keeps its list and tau (the first-tile
+        // seed may sit above its still empty list)
+        if (nc > 0) {
+            if (j < KP) {
+                lv[at(q, j)] = v;
+                lc[at(q, j)] = c;
+            }
+            if (j == KP - 1) tauv[q] = v;
+            if (j == 0) cnt[q] = 0;
         }
-        if (j == KP - 1) tauv[q] = v;
-        if (j == 0) cnt[q] = 0;
     }
 }
 
