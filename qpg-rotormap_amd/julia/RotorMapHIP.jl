@@ -94,10 +94,14 @@ mutable struct HipServer
     W::Int
 end
 
+const POLICIES = Dict(:fifo => Cint(0), :srpt => Cint(1))  # RM_SCHED_FIFO, RM_SCHED_SRPT
+
 function HipServer(index::HipIndex, kmer::Integer, s::Integer, m::Integer, c::Integer;
                    k::Integer = 20, kprime::Integer = 32, mode::Symbol = :fp8_rescore,
-                   max_rows::Integer = 2^14, n_streams::Integer = 2, hbm_budget::Integer = 0)
+                   max_rows::Integer = 2^14, n_streams::Integer = 2, hbm_budget::Integer = 0,
+                   policy::Symbol = :fifo)
     haskey(MODES, mode) || throw(ArgumentError("mode must be :fp8, :fp8_rescore or :fp16"))
+    haskey(POLICIES, policy) || throw(ArgumentError("policy must be :fifo or :srpt"))
     out = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:rm_server_create, LIB), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint, Cint, Cint, Int64, Cint,
@@ -107,6 +111,8 @@ function HipServer(index::HipIndex, kmer::Integer, s::Integer, m::Integer, c::In
     srv = HipServer(out[], index, Int(k), cld(Int(kmer), 16))
     finalizer(v -> (v.ptr != C_NULL && ccall((:rm_server_destroy, LIB), Cint, (Ptr{Cvoid},), v.ptr);
                     v.ptr = C_NULL), srv)
+    # :srpt fills each launch with the requests that have the fewest rows left
+    check(ccall((:rm_server_set_policy, LIB), Cint, (Ptr{Cvoid}, Cint), srv.ptr, POLICIES[policy]))
     return srv
 end
 

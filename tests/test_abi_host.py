@@ -35,6 +35,18 @@ def test_every_header_function_is_exported_and_bound():
     assert not extra, f"bound but not declared: {extra}"
 
 
+def test_julia_glue_calls_declared_functions():
+    """Julia is not in this image, so the glue cannot run here: every symbol it
+    ccalls must at least be declared in the header and exported."""
+    glue = os.path.join(ROOT, "qpg-rotormap_amd", "julia", "RotorMapHIP.jl")
+    called = sorted(set(re.findall(r"ccall\(\(:(rm_\w+),", open(glue).read())))
+    assert len(called) >= 10, called
+    declared = set(header_functions())
+    assert not [n for n in called if n not in declared]
+    lib = C.CDLL(L.LIB_PATH)
+    assert all(hasattr(lib, n) for n in called)
+
+
 def test_version_and_status_codes():
     assert b"gfx950" in L.lib().rm_version()
     text = open(HEADER).read()
