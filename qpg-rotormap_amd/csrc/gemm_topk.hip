@@ -514,9 +514,10 @@ __device__ __forceinline__ int opaque_uniform(int v) {
 }
 
 // LDS index of entry j of query q's list: rows of LP = k' + 1 entries, or
-// (ROT, lists staged in the ring, k' <= 32) rows of 32 entries rotated by q,
-// so that a warp-wide step of the one-thread-per-query merge (same j,
-// consecutive q) touches 64 different banks.
+// (ROT, lists staged in the ring, k' <= 32) rows of 32 entries rotated by q.
+// A wave's access is then free of bank conflicts both with one lane per entry
+// of two consecutive queries (staging, the lane merge) and with one lane per
+// query at the same j.
 template <bool ROT>
 struct ListAt {
     int LP;
