@@ -47,6 +47,29 @@ def test_julia_glue_calls_declared_functions():
     assert all(hasattr(lib, n) for n in called)
 
 
+def test_header_and_integration_example_compile_as_c(tmp_path):
+    """The header is plain C (C callers and FFI generators read it), and the C
+    example in INTEGRATION.md compiles against it with no implicit
+    declarations or pointer mismatches."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```c\n(.*?)```", text, re.S)
+    assert blocks, "INTEGRATION.md has no C example"
+    body = "\n".join(l for l in blocks[0].splitlines() if not l.startswith("#include"))
+    src = tmp_path / "example.c"
+    src.write_text("#include <stddef.h>\n#include <stdint.h>\n#include \"rotormap_hip.h\"\n"
+                   "void example(const uint8_t *host_B, int64_t N, const uint32_t *d_words,\n"
+                   "             int64_t n, float *d_vals, int32_t *d_locs) {\n" + body + "\n}\n")
+    r = subprocess.run([gcc, "-std=c11", "-fsyntax-only", "-Wall", "-Werror",
+                        "-I", os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_version_and_status_codes():
     assert b"gfx950" in L.lib().rm_version()
     text = open(HEADER).read()
