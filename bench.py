@@ -366,9 +366,10 @@ def main():
         engu = R.TopKEngine(eng8.index, k=20, kprime=32, mode="fp8_rescore", rows_cap=total)
         tt = time.perf_counter()
         buckets = 2 ** np.arange(0, 18)
-        best, qg, ms = engu.autotune(buckets)
+        tile, best, qg, ms = engu.autotune(buckets)
         t_tune = time.perf_counter() - tt
-        log(f"autotune: {len(buckets)} buckets in {t_tune:.1f} s; splits {best.tolist()}")
+        log(f"autotune: {len(buckets)} buckets in {t_tune:.1f} s; tiles {tile.tolist()} "
+            f"splits {best.tolist()}")
         torch.cuda.synchronize()
         t = time.perf_counter()
         for b in range(len(sizes)):
@@ -386,7 +387,8 @@ def main():
             "one_launch_per_batch_reads_s": rnd(total / t_each, 1),
             "single_persistent_launch_reads_s": rnd(total / t_one, 1),
             "autotune_s": rnd(t_tune, 2),
-            "autotuned_splits": {int(b): int(s) for b, s in zip(buckets, best)},
+            "autotuned_plans": {int(b): [int(t), int(s), int(g)]
+                                for b, t, s, g in zip(buckets, tile, best, qg)},
             "recall_top20": rnd(ru, 4)}
         log(f"uneven: per-batch {total / t_each:.0f} reads/s, one launch {total / t_one:.0f} reads/s")
         del engu, uw

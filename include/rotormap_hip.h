@@ -162,26 +162,40 @@ int rm_map_batch(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_
 
 /* Uneven batches in ONE launch: batch b holds rows [offs[b], offs[b+1]) of
  * the concatenated words (host offs, nb+1 entries); results land at the same
- * rows of vals/locs.  The persistent kernel walks a work list of
- * (row tile, column split) items across all batches, with tile shapes and
- * split counts from the context's tile plan (rm_autotune). */
+ * rows of vals/locs.  Rows are independent, so the batches are encoded
+ * together and the persistent kernel walks one work list of (row tile,
+ * column split) items across all of them, with the tile shape and split
+ * count the context's tile plan gives the total row count (rm_autotune).
+ * Each row's result equals that of rm_map_batch on its batch alone. */
 int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words,
                    const int64_t *offs, int nb, int k, int s, int m, int c, int ktop, int kprime,
                    int mode, float *vals, int32_t *locs, int64_t ldo, void *stream);
 
 /* ---- tile plan / autotuner ------------------------------------------------
- * A plan is (column splits S, query tiles grouped per XCD qg) per batch-size
- * bucket ceil(log2 M).  rm_plan_splits returns the split count a batch of M
- * rows will use.  rm_autotune times the candidate plans of each bucket on
- * `idx` (tuning queries: the index's own columns) and keeps the fastest in
- * the context (host arrays of nbuckets entries: bucket row counts in; chosen
- * S, qg and ms out). */
-int rm_plan_splits(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode);
-/* Pin the plan for batches in M's bucket (ceil(log2 M)): the column split
- * count (0 clears the entry) and the query tiles grouped per XCD (0 = all). */
-int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int splits, int qgroup);
+ * A plan is (query rows per tile 64 / 128 / 256, column splits S, query
+ * tiles grouped per XCD qg) per (mode, batch-size bucket ceil(log2 M)); the
+ * reference's fixed chunk width w and rows_cap (RotorMap/search/
+ * engine_fp8.jl:309-337) are what it replaces.  Without a tuned entry a
+ * batch takes the smallest tile that holds it (256 from 129 rows) and the
+ * built-in split rule; lists longer than 32 (kp > 32) always run 128-row
+ * tiles.  Results never depend on the plan.
+ * rm_plan_get: the plan a batch of M rows gets; kp is the list length the
+ * fused kernel keeps (kprime for RM_MODE_FP8_RESCORE, else ktop). */
+int rm_plan_get(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int32_t *tile_rows,
+                int32_t *splits, int32_t *qgroup);
+/* Pin the plan for batches in M's bucket: tile_rows 0 (the default tile),
+ * 64, 128 or 256; splits 0 (the default rule) .. 256; qgroup 0 (all) or the
+ * query tiles grouped per XCD.  tile_rows = splits = 0 clears the entry. */
+int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int tile_rows, int splits, int qgroup);
+/* Autotuner: times the candidate plans of each bucket on `idx` (tile rows
+ * up to the first that holds the bucket, split counts around one wave of
+ * workgroups, XCD groupings; tuning queries: the index's own columns, at
+ * this ktop / kprime) and keeps the fastest in the context.  Host arrays of
+ * nbuckets entries: bucket row counts in; chosen tile rows, S, qg and the
+ * mean ms per launch out (any may be NULL). */
 int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, int nbuckets,
-                int mode, int32_t *best_splits, int32_t *best_qgroup, float *best_ms);
+                int mode, int ktop, int kprime, int32_t *best_tile, int32_t *best_splits,
+                int32_t *best_qgroup, float *best_ms);
 
 /* Launch shape of the fused kernels (no counterpart in the reference): on
  * (the default), a grid of one workgroup per CU walks the (row tile, split)
@@ -202,7 +216,11 @@ int rm_prof_read(rm_ctx *ctx, double *kernel_ms, int64_t *launches, double *flop
 /* ---- admission model -------------------------------------------------------
  * Device bytes a batch of M rows needs beyond the resident index (queries,
  * fp32 rescoring copies, split partials, outputs); the resident-index rule
- * of RotorMap/search/engine_compact.jl:195-242 restated for the fused path. */
+ * of RotorMap/search/engine_compact.jl:195-242 restated for the fused path.
+ * It takes no plan, so the split partials are counted for the largest split
+ * count (256): a worst-case upper bound, far above what a large batch uses
+ * (a 65,536-row batch runs one split).  rm_server_create admits by the
+ * actual plans instead. */
 int64_t rm_batch_bytes(int64_t M, int k, int rdim, int ktop, int kprime, int mode);
 /* Free device memory reported by the HIP runtime, bytes. */
 int64_t rm_device_free_bytes(rm_ctx *ctx);
@@ -228,7 +246,10 @@ int rm_server_submit_ref(rm_server *srv, const uint32_t *host_words, int64_t n, 
 /* Poll / wait for a finished request: copies (n x ktop) row-major vals/locs
  * to host buffers.  timeout_ms < 0 waits forever; returns RM_ERR_STATE if
  * not finished within the timeout.  t_* (may be NULL): submit, admit and
- * finish times in seconds on the server's monotonic clock. */
+ * finish times in seconds on the server's monotonic clock.  A failed request
+ * returns only once none of its rows is in flight (its submit_ref words are
+ * then free to reuse).  A request id has one waiter: the call that returns
+ * releases it. */
 int rm_server_wait(rm_server *srv, int64_t req_id, int timeout_ms, float *host_vals,
                    int32_t *host_locs, double *t_submit, double *t_admit, double *t_finish);
 /* Cancel a request that has not been admitted yet. */

@@ -7,7 +7,7 @@
  *
  * Two encoders live here:
  *   rmo_ref_rope_frag_real  the FP64 golden encoder, restated line by line
- *                           from RotorMap/encode/reference.jl:67-135
+ *                           from RotorMap/encode/reference.jl:37-105
  *                           (cis(theta*p) in FP64, no recurrence);
  *   rmo_rope_fixed          the deterministic fixed-point encoder the HIP
  *                           kernel implements (integer phasor table, int64
@@ -35,7 +35,7 @@
 /* ------------------------------------------------------------------------ */
 
 /* Base j (0-based) sits in word j>>4 at bit 2*(j&15).
- * RotorMap/encode/reference.jl:59-65 (_frag_codes). */
+ * RotorMap/encode/reference.jl:29-35 (_frag_codes). */
 void rmo_frag_codes(const uint32_t *words, int k, uint8_t *codes) {
     for (int j = 0; j < k; ++j)
         codes[j] = (uint8_t)((words[j >> 4] >> (2 * (j & 15))) & 3u);
@@ -124,7 +124,7 @@ void rmo_revcomp_codes(const uint8_t *codes, int k, uint8_t *rc) {
 /* ------------------------------------------------------------------------ */
 
 /* Compact key (0-based csmer) of the window at 0-based p.
- * RotorMap/encode/reference.jl:73-80: little-endian s-mer field,
+ * RotorMap/encode/reference.jl:43-50: little-endian s-mer field,
  * lower ^ ((upper * 0x9E3779B1 mod 2^32) & (4^c-1)). */
 static uint32_t rmo_key(const uint8_t *codes, int p, int s, int c) {
     uint32_t smer = 0;
@@ -136,7 +136,7 @@ static uint32_t rmo_key(const uint8_t *codes, int p, int s, int c) {
 }
 
 /* Integer s-mer histogram: counts[csmer] = windows whose compact key is csmer
- * (before any G-bin zeroing; the bin bookkeeping of reference.jl:72-85). */
+ * (before any G-bin zeroing; the bin bookkeeping of reference.jl:43-55). */
 void rmo_smer_counts(const uint8_t *codes, int k, int s, int c, int32_t *counts) {
     int nbin = 1 << (2 * c);
     memset(counts, 0, sizeof(int32_t) * (size_t)nbin);
@@ -145,7 +145,7 @@ void rmo_smer_counts(const uint8_t *codes, int k, int s, int c, int32_t *counts)
 }
 
 /* 0-based compact key of the zeroed G-homopolymer bin: 2 * sum_{i<c} 4^i.
- * RotorMap/encode/reference.jl:86-90. */
+ * RotorMap/encode/reference.jl:56-60. */
 static int rmo_gbin(int c) {
     int allones = 0;
     for (int i = 0; i < c; ++i) allones += 1 << (2 * i);
@@ -156,7 +156,7 @@ static int rmo_gbin(int c) {
 /* FP64 golden encoder (the reference's CPU golden path)                     */
 /* ------------------------------------------------------------------------ */
 
-/* RotorMap/encode/reference.jl:67-135 (_ref_rope_frag + _ref_rope_frag_real).
+/* RotorMap/encode/reference.jl:37-105 (_ref_rope_frag :37-93 + _ref_rope_frag_real :96-105).
  * out[2D] = [Re(b_0..b_{D-1}) ; Im(...)], norms[m].  theta = f * (2pi/k),
  * f = 1 (m == 1) else 2*idm/(m-1) + 1 (0-based idm); hist[b] += cis(theta*p). */
 void rmo_ref_rope_frag_real(const uint8_t *codes, int k, int s, int m, int c,
@@ -213,7 +213,7 @@ void rmo_ref_rope_frag_real(const uint8_t *codes, int k, int s, int m, int c,
 
 /* Phasor table: tab[2*(idm*nw + p) + {0,1}] = round(2^30 * {cos,sin}(2pi*q)),
  * q = frac(f*p/k) formed exactly in integers: f*p/k = (2*idm + m-1)*p /
- * ((m-1)*k) for m > 1, p/k for m == 1 (the phase of reference.jl:82-83). */
+ * ((m-1)*k) for m > 1, p/k for m == 1 (the phase of reference.jl:52-53). */
 void rmo_phase_table(int k, int s, int m, int32_t *tab) {
     int nw = k - s + 1;
     for (int idm = 0; idm < m; ++idm) {
@@ -493,7 +493,7 @@ void rmo_rescore_rank(const float *q, const uint8_t *B, int K, const int32_t *ca
 /* The reference's CPU golden mapping path, threaded (cpu_baseline only)      */
 /* ------------------------------------------------------------------------ */
 
-/* Per read: FP64 golden encoder (reference.jl:67-135) -> the query storage
+/* Per read: FP64 golden encoder (reference.jl:37-105) -> the query storage
  * chain e4m3(fp32(fp16(.))) (engine_fp8.jl:365-368) -> FP64 dot products
  * against every dequantised e4m3 column (engine_fp8.jl:873-876) -> top-ktop
  * by (score desc, column asc) (engine_fp8.jl:875).  OpenMP over reads. */

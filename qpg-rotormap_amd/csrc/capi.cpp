@@ -84,8 +84,8 @@ int table_get(rm_ctx *ctx, int k, int s, int m, const int32_t **out) {
 // a working set past the MALL sends those re-reads to HBM (65,536 fp16
 // queries of 4 KB: 256 MB at S = 1, 19 % slower than S = 2 at 3.1 M columns;
 // the e4m3 batch, 128 MB, is fastest at S = 1).
-int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp, int64_t row_bytes) {
-    const int64_t BR = rows_per_tile(kp);
+int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int bq, int64_t row_bytes) {
+    const int64_t BR = bq;
     const int64_t R = cdiv(M, BR), tiles = cdiv(N, kBI), cus = ctx->num_cus;
     const int64_t cap = std::min<int64_t>(256, tiles);
     int64_t S = std::max<int64_t>(1, std::min<int64_t>(cus / R, cap));
@@ -100,19 +100,35 @@ static int bucket_of(int64_t M) {
     return b;  // ceil(log2 M)
 }
 
-// Tile plan for a batch: (splits, query tiles per XCD group).  The built-in
-// rule is default_splits with one group over all query tiles; the autotuned
-// plan (rm_autotune / rm_plan_set) overrides it per size bucket.
-std::pair<int, int> plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
-    auto it = ctx->plan.find({mode == RM_MODE_FP16 ? 1 : 0, bucket_of(M)});
-    int64_t tiles = cdiv(idx->N, kBI);
-    if (it != ctx->plan.end())
-        return {(int)std::max<int64_t>(1, std::min<int64_t>(it->second.first, tiles)), it->second.second};
-    return {default_splits(ctx, M, idx->N, kp, idx->rdim * (mode == RM_MODE_FP16 ? 2 : 1)), 0};
+// Query rows per tile of a batch with no tuned plan: the smallest ring tile
+// that holds the batch (a small batch wastes less MFMA work on padded query
+// rows), 256 from 129 rows up; lists longer than 32 have only the 128 tile.
+static int default_tile(int64_t M, int kp) {
+    if (!ring_list(kp)) return 128;
+    return M <= 64 ? 64 : M <= 128 ? 128 : 256;
 }
 
-static int splits_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
-    return plan_for(ctx, idx, M, mode, kp).first;
+// Tile plan for a batch: (query rows per tile, splits, query tiles per XCD
+// group).  The built-in rule is default_tile + default_splits with one group
+// over all query tiles; the autotuned plan (rm_autotune / rm_plan_set)
+// overrides it per (mode, size bucket).  A tuned tile a list length cannot
+// run falls back to that length's tile.
+Plan plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
+    const int64_t rb = idx->rdim * (mode == RM_MODE_FP16 ? 2 : 1);
+    const int64_t tiles = cdiv(idx->N, kBI);
+    Plan p;
+    auto it = ctx->plan.find({mode, bucket_of(M)});
+    if (it != ctx->plan.end()) {
+        p = it->second;
+        if (p.bq == 0 || !tile_rows_ok(kp, p.bq)) p.bq = default_tile(M, kp);
+        if (p.S <= 0) p.S = default_splits(ctx, M, idx->N, p.bq, rb);
+        p.S = (int)std::max<int64_t>(1, std::min<int64_t>(p.S, tiles));
+        return p;
+    }
+    p.bq = default_tile(M, kp);
+    p.S = default_splits(ctx, M, idx->N, p.bq, rb);
+    p.qg = 0;
+    return p;
 }
 
 static int clamp_splits(const rm_index *idx, int S) {
@@ -124,24 +140,31 @@ size_t part_bytes(const rm_index *idx, int64_t M, int KP, int S) {
     return ((size_t)M * clamp_splits(idx, S) * KP + (size_t)M) * 4;
 }
 
+int max_splits_within(const rm_index *idx, int64_t M, int KP, int S, size_t pbytes) {
+    S = clamp_splits(idx, S);
+    if (M <= 0) return S;
+    const int64_t fit = ((int64_t)(pbytes / 4) - M) / (M * (int64_t)KP);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(S, fit));
+}
+
 int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
               const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
-              float *vals, int32_t *locs, int64_t ldo, int S, int qg, hipStream_t st) {
+              float *vals, int32_t *locs, int64_t ldo, const Plan &pl, hipStream_t st) {
     const int KP = (mode == RM_MODE_FP8_RESCORE) ? kprime : ktop;
     if (M == 0) return RM_OK;
-    const size_t pbytes = part_bytes(idx, M, KP, S);
+    const size_t pbytes = part_bytes(idx, M, KP, pl.S);
     void *pv = nullptr, *pc = nullptr;
     int rc;
     if ((rc = ws_get(ctx, "part_v", pbytes, &pv))) return rc;
     if ((rc = ws_get(ctx, "part_c", pbytes, &pc))) return rc;
     return topk_core_bufs(ctx, idx, Q, ldq_bytes, elem, Q32, ldq32, M, ktop, kprime, mode, vals,
-                          locs, ldo, S, qg, (float *)pv, (int32_t *)pc, pbytes, st);
+                          locs, ldo, pl, (float *)pv, (int32_t *)pc, pbytes, st);
 }
 
 int topk_core_bufs(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
                    const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
-                   float *vals, int32_t *locs, int64_t ldo, int S, int qg, float *pv, int32_t *pc,
-                   size_t pbytes, hipStream_t st) {
+                   float *vals, int32_t *locs, int64_t ldo, const Plan &pl, float *pv,
+                   int32_t *pc, size_t pbytes, hipStream_t st) {
     const int KP = (mode == RM_MODE_FP8_RESCORE) ? kprime : ktop;
     const int64_t K = idx->rdim;
     const int esz = elem == 0 ? 1 : 2;
@@ -150,7 +173,10 @@ int topk_core_bufs(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_
              elem == 0 ? "e4m3" : "fp16");
     if (M == 0) return RM_OK;
     const int64_t tiles = cdiv(idx->N, kBI);
-    S = clamp_splits(idx, S);
+    const int S = clamp_splits(idx, pl.S);
+    const int bq = pl.bq;
+    RM_CHECK(tile_rows_ok(KP, bq), RM_ERR_ARG, "no fused kernel for %d queries per tile at k' = %d",
+             bq, KP);
     RM_CHECK(part_bytes(idx, M, KP, S) <= pbytes, RM_ERR_STATE,
              "split partials need %zu bytes, workspace holds %zu", part_bytes(idx, M, KP, S), pbytes);
     int rc;
@@ -160,11 +186,12 @@ int topk_core_bufs(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_
     a.q = (const uint8_t *)Q;
     a.ldq = ldq_bytes;
     a.M = M;
+    a.BQ = bq;
     a.N = idx->N;
     a.Kb = (int)(K * esz);
     a.S = S;
-    a.R = cdiv(M, rows_per_tile(KP));
-    a.qg = (qg <= 0 || qg > a.R) ? (int)a.R : qg;
+    a.R = cdiv(M, bq);
+    a.qg = (pl.qg <= 0 || pl.qg > a.R) ? (int)a.R : pl.qg;
     a.ngroups = (int)cdiv(a.R, a.qg);
     a.tiles = tiles;
     a.col_base = 0;
@@ -391,6 +418,7 @@ int rm_gemm_chunk(rm_ctx *ctx, void *C, int c_dtype, int64_t ldc, const void *A,
     a.q = q;
     a.ldq = ldq;
     a.M = M;
+    a.BQ = 256;
     a.N = wi;
     a.Kb = (int)(K * esz);
     a.R = cdiv(M, 256);
@@ -510,9 +538,9 @@ int rm_topk_batch(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq, 
     if (mode == RM_MODE_FP8_RESCORE) RM_CHECK(Q32 && ldq32 >= idx->rdim, RM_ERR_ARG, "Q32 required");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     RM_HIP(hipSetDevice(ctx->device));
-    const auto pl = plan_for(ctx, idx, M, mode, mode == RM_MODE_FP8_RESCORE ? kprime : ktop);
+    const Plan pl = plan_for(ctx, idx, M, mode, mode == RM_MODE_FP8_RESCORE ? kprime : ktop);
     return topk_core(ctx, idx, Q, ldq_b, elem, Q32, ldq32, M, ktop, kprime, mode, vals, locs, ldo,
-                     pl.first, pl.second, pick_stream(ctx, stream));
+                     pl, pick_stream(ctx, stream));
 }
 
 // encode n fragments straight into the query layouts the GEMM reads
@@ -572,9 +600,9 @@ int rm_map_batch(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_
     float *q32;
     if ((rc = encode_queries(ctx, words, n, k, s, m, c, mode, rdim, norms, &q, &q32, st))) return rc;
     const int elem = mode == RM_MODE_FP16 ? 1 : 0;
-    const auto pl = plan_for(ctx, idx, n, mode, mode == RM_MODE_FP8_RESCORE ? kprime : ktop);
+    const Plan pl = plan_for(ctx, idx, n, mode, mode == RM_MODE_FP8_RESCORE ? kprime : ktop);
     return topk_core(ctx, idx, q, rdim * (elem ? 2 : 1), elem, q32, rdim, n, ktop, kprime, mode,
-                     vals, locs, ldo, pl.first, pl.second, st);
+                     vals, locs, ldo, pl, st);
 }
 
 int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, const int64_t *offs,
@@ -593,37 +621,66 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, cons
                         nullptr, stream);
 }
 
-int rm_plan_splits(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode) {
-    if (!ctx || !idx || M <= 0) return 0;
+int rm_plan_get(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int32_t *tile_rows,
+                int32_t *splits, int32_t *qgroup) {
+    RM_CHECK(ctx && idx && M >= 1, RM_ERR_ARG, "NULL argument or M < 1");
+    RM_CHECK(kp >= 1 && kp <= kMaxKP, RM_ERR_ARG, "kp must be 1..%d", kMaxKP);
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    return splits_for(ctx, idx, M, mode, mode == RM_MODE_FP8_RESCORE ? 32 : 20);
-}
-
-int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int splits, int qgroup) {
-    RM_CHECK(ctx && M >= 1 && splits >= 0 && splits <= 256 && qgroup >= 0, RM_ERR_ARG,
-             "bad plan entry");
-    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    auto key = std::make_pair(mode == RM_MODE_FP16 ? 1 : 0, bucket_of(M));
-    if (splits == 0) ctx->plan.erase(key);
-    else ctx->plan[key] = {splits, qgroup};
+    const Plan p = plan_for(ctx, idx, M, mode, kp);
+    if (tile_rows) *tile_rows = p.bq;
+    if (splits) *splits = p.S;
+    if (qgroup) *qgroup = p.qg;
     return RM_OK;
 }
 
+int rm_plan_set(rm_ctx *ctx, int64_t M, int mode, int tile_rows, int splits, int qgroup) {
+    RM_CHECK(ctx && M >= 1 && splits >= 0 && splits <= 256 && qgroup >= 0, RM_ERR_ARG,
+             "bad plan entry");
+    RM_CHECK(mode == RM_MODE_FP8 || mode == RM_MODE_FP8_RESCORE || mode == RM_MODE_FP16,
+             RM_ERR_ARG, "bad mode %d", mode);
+    RM_CHECK(tile_rows == 0 || tile_rows == 64 || tile_rows == 128 || tile_rows == 256, RM_ERR_ARG,
+             "tile_rows must be 0 (default), 64, 128 or 256 (got %d)", tile_rows);
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    auto key = std::make_pair(mode, bucket_of(M));
+    if (splits == 0 && tile_rows == 0) {
+        ctx->plan.erase(key);
+    } else {
+        Plan p;
+        p.bq = tile_rows;  // 0: the list length's default tile
+        p.S = splits;      // 0: the default split rule for that tile
+        p.qg = qgroup;
+        ctx->plan[key] = p;
+    }
+    return RM_OK;
+}
+
+namespace {
+// a pair of timing events, destroyed on every return path
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~EventPair() {
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+    }
+};
+}  // namespace
+
 int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, int nbuckets,
-                int mode, int32_t *best_splits, int32_t *best_qgroup, float *best_ms) {
+                int mode, int ktop, int kprime, int32_t *best_tile, int32_t *best_splits,
+                int32_t *best_qgroup, float *best_ms) {
     RM_CHECK(ctx && idx && bucket_rows && nbuckets >= 0, RM_ERR_ARG, "NULL argument");
     int rc;
-    if ((rc = check_query(idx, 20, 32, mode))) return rc;
+    if ((rc = check_query(idx, ktop, kprime, mode))) return rc;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     RM_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int elem = mode == RM_MODE_FP16 ? 1 : 0;
     const int64_t rdim = idx->rdim;
     const int esz = elem ? 2 : 1;
-    const int kp = mode == RM_MODE_FP8_RESCORE ? 32 : 20;
-    hipEvent_t e0, e1;
-    RM_HIP(hipEventCreate(&e0));
-    RM_HIP(hipEventCreate(&e1));
+    const int kp = mode == RM_MODE_FP8_RESCORE ? kprime : ktop;
+    EventPair ev;
+    RM_HIP(hipEventCreate(&ev.e0));
+    RM_HIP(hipEventCreate(&ev.e1));
     for (int b = 0; b < nbuckets; ++b) {
         const int64_t M = bucket_rows[b];
         RM_CHECK(M >= 1, RM_ERR_ARG, "bucket rows must be >= 1");
@@ -631,8 +688,8 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
         if ((rc = ws_get(ctx, "tune_q", (size_t)M * rdim * esz, &q))) return rc;
         if (mode == RM_MODE_FP8_RESCORE && (rc = ws_get(ctx, "tune_q32", (size_t)M * rdim * 4, &q32)))
             return rc;
-        if ((rc = ws_get(ctx, "tune_v", (size_t)M * 20 * 4, &vo))) return rc;
-        if ((rc = ws_get(ctx, "tune_l", (size_t)M * 20 * 4, &lo))) return rc;
+        if ((rc = ws_get(ctx, "tune_v", (size_t)M * ktop * 4, &vo))) return rc;
+        if ((rc = ws_get(ctx, "tune_l", (size_t)M * ktop * 4, &lo))) return rc;
         // tuning queries: index columns themselves (unit-energy embeddings
         // with one perfect match each, like reads), repeated when M > N
         const uint8_t *src = elem ? (const uint8_t *)idx->b16 : idx->b8;
@@ -642,51 +699,60 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
                                   hipMemcpyDeviceToDevice, st));
         }
         if (q32 && (rc = launch_dequantize((float *)q32, (const uint8_t *)q, M * rdim, st))) return rc;
-        const int64_t R = cdiv(M, rows_per_tile(kp));
         // time each candidate on a prefix of the index sized to ~4e12 FLOP per
         // launch (a few ms), at least 256 tiles: the plan depends on the row
-        // count, while the per-column work only scales the time
+        // count, while the per-column work only scales the time (small
+        // batches, HBM-bound, run over the whole index)
         rm_index sub = *idx;
         const int64_t ncap = std::max<int64_t>(256LL * kBI, (int64_t)(4e12 / (2.0 * M * rdim)));
         sub.N = std::min<int64_t>(idx->N, ncap);
         const int64_t stiles = cdiv(sub.N, kBI);
-        // splits from a quarter of one wave of workgroups to four waves
-        const int64_t wave = std::max<int64_t>(1, ctx->num_cus / R);
-        std::vector<std::pair<int, int>> cands;
-        for (int S = 1; S <= 256 && S <= stiles; S *= 2) {
-            if (4 * S < wave && 2 * S <= std::min<int64_t>(256, stiles)) continue;
-            if (R * S > 4LL * ctx->num_cus && S > 1) break;
-            for (int qg : {0, 8, 4, 2}) {
-                if (qg && qg >= R) continue;
-                cands.push_back({S, qg});
+        // tiles: every ring tile up to the first that holds the batch; splits
+        // from a quarter of one wave of workgroups to four waves
+        std::vector<Plan> cands;
+        for (int bq : {64, 128, 256}) {
+            if (!tile_rows_ok(kp, bq)) continue;
+            if (bq > 64 && bq / 2 >= M) continue;
+            const int64_t R = cdiv(M, bq);
+            const int64_t wave = std::max<int64_t>(1, ctx->num_cus / R);
+            for (int S = 1; S <= 256 && S <= stiles; S *= 2) {
+                if (4 * S < wave && 2 * S <= std::min<int64_t>(256, stiles)) continue;
+                if (R * S > 4LL * ctx->num_cus && S > 1) break;
+                for (int qg : {0, 8, 4, 2}) {
+                    if (qg && qg >= R) continue;
+                    Plan p;
+                    p.bq = bq;
+                    p.S = S;
+                    p.qg = qg;
+                    cands.push_back(p);
+                }
             }
         }
-        std::pair<int, int> best = cands.empty() ? std::make_pair(1, 0) : cands[0];
+        Plan best = cands.empty() ? plan_for(ctx, &sub, M, mode, kp) : cands[0];
         float bestT = 1e30f;
-        for (auto c : cands) {
+        for (const Plan &c : cands) {
             auto run = [&]() {
-                return topk_core(ctx, &sub, q, rdim * esz, elem, (float *)q32, rdim, M, 20, kp, mode,
-                                 (float *)vo, (int32_t *)lo, M, c.first, c.second, st);
+                return topk_core(ctx, &sub, q, rdim * esz, elem, (float *)q32, rdim, M, ktop, kprime,
+                                 mode, (float *)vo, (int32_t *)lo, M, c, st);
             };
             if ((rc = run())) return rc;  // warm-up
-            RM_HIP(hipEventRecord(e0, st));
+            RM_HIP(hipEventRecord(ev.e0, st));
             const int reps = 2;
             for (int r = 0; r < reps; ++r)
                 if ((rc = run())) return rc;
-            RM_HIP(hipEventRecord(e1, st));
-            RM_HIP(hipEventSynchronize(e1));
+            RM_HIP(hipEventRecord(ev.e1, st));
+            RM_HIP(hipEventSynchronize(ev.e1));
             float ms = 0;
-            RM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            RM_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
             ms /= reps;
             if (ms < bestT) { bestT = ms; best = c; }
         }
-        ctx->plan[{elem, bucket_of(M)}] = best;
-        if (best_splits) best_splits[b] = best.first;
-        if (best_qgroup) best_qgroup[b] = best.second;
+        ctx->plan[{mode, bucket_of(M)}] = best;
+        if (best_tile) best_tile[b] = best.bq;
+        if (best_splits) best_splits[b] = best.S;
+        if (best_qgroup) best_qgroup[b] = best.qg;
         if (best_ms) best_ms[b] = bestT;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     return RM_OK;
 }
 

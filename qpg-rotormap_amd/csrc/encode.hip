@@ -2,7 +2,7 @@
 // word kernels around it (reverse complement, window slicing, conversions).
 //
 // Semantics follow RotorMap/encode/kernel.jl:86-290 (kernel_rope_frag_real)
-// and its CPU golden twin RotorMap/encode/reference.jl:67-135:
+// and its CPU golden twin RotorMap/encode/reference.jl:37-105:
 //   window p (0-based, p < nw = k-s+1) -> little-endian s-mer field of bases
 //   [p, p+s); compact key = lower ^ ((upper * 0x9E3779B1) & (4^c-1));
 //   bin = key*m + idm gets the phasor exp(+i*theta_idm*p),

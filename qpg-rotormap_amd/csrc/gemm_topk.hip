@@ -50,21 +50,42 @@ typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
-// Tile: 256 index rows x BR queries, BR = 64*BB (BB = 4: 256, BB = 2: 128).
-// Four waves in 2 x 2; a wave owns 128 index rows x 32*BB queries as 4 x BB
-// blocks of 32 x 32.
-template <int WR, int BB>
-struct Tile {
-    static constexpr int NT = 128 * WR;          // threads: 2 x WR waves
-    static constexpr int BR = 32 * WR * BB;      // queries per tile
+// Tile shape: kBI = 256 index rows x BR queries over NT / 64 waves.  A wave
+// owns 64 queries (BB = 2 blocks of 32) and 32 * AI index rows (AI blocks):
+// the waves form WI index groups x WQ = BR / 64 query groups, wave w at index
+// group w % WI and query group w / WI.  Every shape gives a query 16 groups
+// of 16 accumulators (AI per lane x 2 lanes per wave x WI waves), which the
+// first-tile bound relies on.
+//   Shape<512, 256, true>   8 waves, 2 x 4, wave tile 128 x 64 (ring)
+//   Shape<512, 128, true>   8 waves, 4 x 2, wave tile  64 x 64 (ring)
+//   Shape<512,  64, true>   8 waves, 8 x 1, wave tile  32 x 64 (ring)
+//   Shape<256, 128, false>  4 waves, 2 x 2, wave tile 128 x 64 (register staging)
+//   Shape<512, 256, false>  the C-writing kernel (register staging)
+template <int NT_, int BR_, bool RING_>
+struct Shape {
+    static constexpr int NT = NT_;               // threads
+    static constexpr int NW = NT / 64;           // waves
+    static constexpr int BR = BR_;               // queries per tile
+    static constexpr int BB = 2;                 // 32-query blocks per wave
+    static constexpr int WQ = BR / 64;           // query groups
+    static constexpr int WI = NW / WQ;           // index groups
+    static constexpr int AI = kBI / 32 / WI;     // 32-row index blocks per wave
+    static constexpr bool RING = RING_;
     static constexpr int ROWS = kBI + BR;        // LDS rows per stage buffer
     static constexpr int CHUNKS = ROWS * 4 / NT; // 16-byte chunks per thread per stage
     static constexpr int STAGE_BYTES = ROWS * kRow;
+    static_assert(WQ >= 1 && WI * WQ == NW && AI * 32 * WI == kBI, "tile shape");
+    static_assert(2 * AI * WI == 16, "16 groups of 16 accumulators per query");
 };
+using ShapeTopk256 = Shape<512, 256, true>;
+using ShapeTopk128 = Shape<512, 128, true>;
+using ShapeTopk64 = Shape<512, 64, true>;
+using ShapeList128 = Shape<256, 128, false>;
+using ShapeStore = Shape<512, 256, false>;
 
-template <int WR, int BB>
+template <class SH>
 struct StageRegs {
-    uint4 v[Tile<WR, BB>::CHUNKS];
+    uint4 v[SH::CHUNKS];
 };
 
 // Buffer descriptors of a tile's operand rows: base at the tile's first row,
@@ -87,13 +108,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const uint8_t *base,
 // global -> registers for one 64-byte K stage: chunk j of thread tid covers
 // row tid/4 + (NT/4)*j (index rows first, then query rows), bytes
 // (tid%4)*16 .. +16 of the stage.
-template <int WR, int BB>
-__device__ __forceinline__ void g_load(StageRegs<WR, BB> &st, const TileSrc &ts, int tid, int kb) {
-    constexpr int RS = Tile<WR, BB>::NT / 4;  // rows per chunk step
+template <class SH>
+__device__ __forceinline__ void g_load(StageRegs<SH> &st, const TileSrc &ts, int tid, int kb) {
+    constexpr int RS = SH::NT / 4;  // rows per chunk step
+    static_assert(kBI % RS == 0 && SH::BR % RS == 0, "chunk steps split index and query rows");
     const int r = tid >> 2, seg = (tid & 3) * 16;
     const int vdb = r * ts.ldb + seg, vq = r * ts.ldq + seg;
 #pragma unroll
-    for (int j = 0; j < Tile<WR, BB>::CHUNKS; ++j) {
+    for (int j = 0; j < SH::CHUNKS; ++j) {
         if (j < kBI / RS)
             st.v[j] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
                                                     ts.db, vdb, j * RS * ts.ldb + kb, 0));
@@ -110,12 +132,12 @@ __device__ __forceinline__ int phys_chunk(int row, int c) {
     else return c;
 }
 
-template <int WR, int BB>
-__device__ __forceinline__ void s_store(const StageRegs<WR, BB> &st, uint8_t *sb, int tid) {
-    constexpr int RS = Tile<WR, BB>::NT / 4;  // a multiple of 16: same swizzle per chunk j
+template <class SH>
+__device__ __forceinline__ void s_store(const StageRegs<SH> &st, uint8_t *sb, int tid) {
+    constexpr int RS = SH::NT / 4;  // a multiple of 16: same swizzle per chunk j
     uint8_t *p = sb + (tid >> 2) * kRow + phys_chunk(tid >> 2, tid & 3) * 16;
 #pragma unroll
-    for (int j = 0; j < Tile<WR, BB>::CHUNKS; ++j) *(uint4 *)(p + j * RS * kRow) = st.v[j];
+    for (int j = 0; j < SH::CHUNKS; ++j) *(uint4 *)(p + j * RS * kRow) = st.v[j];
 }
 
 __device__ __forceinline__ v8i ld_v8i(const uint8_t *p) {
@@ -130,10 +152,10 @@ __device__ __forceinline__ v8i ld_v8i(const uint8_t *p) {
 // block with K bytes [(l/32)*32, +32) of the stage (fp8), or halves
 // [kk*16 + (l/32)*8, +8) per 16-wide f16 step kk: the same K slice for both
 // operands, so every product pairs matching K indices.
-template <int BB>
+template <int AI>
 struct Frags {
-    uint4 a[4][2];
-    uint4 b[BB][2];
+    uint4 a[AI][2];
+    uint4 b[2][2];
 };
 
 // row = stage buffer + (32-aligned row base + lane % 32) * kRow: the swizzle
@@ -147,19 +169,19 @@ __device__ __forceinline__ void ld_pair(uint4 (&d)[2], const uint8_t *row, int l
     d[1] = *(const uint4 *)(row + phys_chunk(rl, c1) * 16);
 }
 
-template <int ELEM, int BB>
-__device__ __forceinline__ void ld_frags_b(Frags<BB> &f, const uint8_t *sb, int wr, int lane) {
-    // wave wr owns queries [wr*32*BB, (wr+1)*32*BB)
+template <int ELEM, int AI>
+__device__ __forceinline__ void ld_frags_b(Frags<AI> &f, const uint8_t *sb, int wr, int lane) {
+    // wave wr owns queries [wr*64, (wr+1)*64)
 #pragma unroll
-    for (int bj = 0; bj < BB; ++bj)
-        ld_pair<ELEM>(f.b[bj], sb + (kBI + wr * 32 * BB + bj * 32 + (lane & 31)) * kRow, lane);
+    for (int bj = 0; bj < 2; ++bj)
+        ld_pair<ELEM>(f.b[bj], sb + (kBI + wr * 64 + bj * 32 + (lane & 31)) * kRow, lane);
 }
 
-template <int ELEM, int BB, int A0, int A1>
-__device__ __forceinline__ void ld_frags_a(Frags<BB> &f, const uint8_t *sb, int wi, int lane) {
+template <int ELEM, int AI, int A0, int A1>
+__device__ __forceinline__ void ld_frags_a(Frags<AI> &f, const uint8_t *sb, int wi, int lane) {
 #pragma unroll
     for (int ai = A0; ai < A1; ++ai)
-        ld_pair<ELEM>(f.a[ai], sb + (wi * 128 + ai * 32 + (lane & 31)) * kRow, lane);
+        ld_pair<ELEM>(f.a[ai], sb + (wi * (32 * AI) + ai * 32 + (lane & 31)) * kRow, lane);
 }
 
 struct Bytes32 {
@@ -167,13 +189,13 @@ struct Bytes32 {
 };
 
 // FIRST: the tile's first stage, accumulating onto zero (no zeroing pass)
-template <int ELEM, int BB, int A0, int A1, bool FIRST = false>
-__device__ __forceinline__ void mma_part(v16f (&acc)[4][BB], const Frags<BB> &f) {
+template <int ELEM, int AI, int A0, int A1, bool FIRST = false>
+__device__ __forceinline__ void mma_part(v16f (&acc)[AI][2], const Frags<AI> &f) {
     const v16f zero = {};
 #pragma unroll
     for (int ai = A0; ai < A1; ++ai)
 #pragma unroll
-        for (int bj = 0; bj < BB; ++bj) {
+        for (int bj = 0; bj < 2; ++bj) {
             if constexpr (ELEM == 0) {
                 const v8i fa = __builtin_bit_cast(v8i, (Bytes32{f.a[ai][0], f.a[ai][1]}));
                 const v8i fb = __builtin_bit_cast(v8i, (Bytes32{f.b[bj][0], f.b[bj][1]}));
@@ -194,54 +216,56 @@ __device__ __forceinline__ void mma_part(v16f (&acc)[4][BB], const Frags<BB> &f)
 // first two index fragments of stage s, stores stage s+1 (already in
 // registers) into the other buffer and issues stage s+2's global loads; the
 // last two index fragments are read while the first half's MFMAs run.
-template <int ELEM, int WR, int BB>
-__device__ __forceinline__ void tile_product(v16f (&acc)[4][BB], const GemmArgs &a, int64_t i0,
+template <int ELEM, class SH>
+__device__ __forceinline__ void tile_product(v16f (&acc)[SH::AI][2], const GemmArgs &a, int64_t i0,
                                              int64_t r0, uint8_t *sb0, uint8_t *sb1, int tid,
                                              int wi, int wr, int lane) {
+    constexpr int AI = SH::AI, H = (AI + 1) / 2;
     TileSrc ts;
     ts.db = rows_rsrc(a.db, i0, a.N, kBI, a.ldb);
-    ts.q = rows_rsrc(a.q, r0, a.M, Tile<WR, BB>::BR, a.ldq);
+    ts.q = rows_rsrc(a.q, r0, a.M, SH::BR, a.ldq);
     ts.ldb = (int)a.ldb;
     ts.ldq = (int)a.ldq;
     const int nk = a.Kb / kBK;  // >= 1
-    StageRegs<WR, BB> g;
-    Frags<BB> f;
-    g_load<WR, BB>(g, ts, tid, 0);
-    s_store<WR, BB>(g, sb0, tid);
-    if (nk > 1) g_load<WR, BB>(g, ts, tid, kBK);
+    StageRegs<SH> g;
+    Frags<AI> f;
+    g_load<SH>(g, ts, tid, 0);
+    s_store<SH>(g, sb0, tid);
+    if (nk > 1) g_load<SH>(g, ts, tid, kBK);
     {  // stage 0 accumulates onto zero
         __syncthreads();
-        ld_frags_b<ELEM, BB>(f, sb0, wr, lane);
-        ld_frags_a<ELEM, BB, 0, 2>(f, sb0, wi, lane);
-        if (1 < nk) s_store<WR, BB>(g, sb1, tid);
-        if (2 < nk) g_load<WR, BB>(g, ts, tid, 2 * kBK);
-        mma_part<ELEM, BB, 0, 2, true>(acc, f);
-        ld_frags_a<ELEM, BB, 2, 4>(f, sb0, wi, lane);
-        mma_part<ELEM, BB, 2, 4, true>(acc, f);
+        ld_frags_b<ELEM, AI>(f, sb0, wr, lane);
+        ld_frags_a<ELEM, AI, 0, H>(f, sb0, wi, lane);
+        if (1 < nk) s_store<SH>(g, sb1, tid);
+        if (2 < nk) g_load<SH>(g, ts, tid, 2 * kBK);
+        mma_part<ELEM, AI, 0, H, true>(acc, f);
+        ld_frags_a<ELEM, AI, H, AI>(f, sb0, wi, lane);
+        mma_part<ELEM, AI, H, AI, true>(acc, f);
     }
     for (int s = 1; s < nk; ++s) {
         uint8_t *cur = (s & 1) ? sb1 : sb0;
         uint8_t *nxt = (s & 1) ? sb0 : sb1;
         __syncthreads();
-        ld_frags_b<ELEM, BB>(f, cur, wr, lane);
-        ld_frags_a<ELEM, BB, 0, 2>(f, cur, wi, lane);
-        if (s + 1 < nk) s_store<WR, BB>(g, nxt, tid);
-        if (s + 2 < nk) g_load<WR, BB>(g, ts, tid, (s + 2) * kBK);
-        mma_part<ELEM, BB, 0, 2>(acc, f);
-        ld_frags_a<ELEM, BB, 2, 4>(f, cur, wi, lane);
-        mma_part<ELEM, BB, 2, 4>(acc, f);
+        ld_frags_b<ELEM, AI>(f, cur, wr, lane);
+        ld_frags_a<ELEM, AI, 0, H>(f, cur, wi, lane);
+        if (s + 1 < nk) s_store<SH>(g, nxt, tid);
+        if (s + 2 < nk) g_load<SH>(g, ts, tid, (s + 2) * kBK);
+        mma_part<ELEM, AI, 0, H>(acc, f);
+        ld_frags_a<ELEM, AI, H, AI>(f, cur, wi, lane);
+        mma_part<ELEM, AI, H, AI>(acc, f);
     }
 }
 
 // ---------------------------------------------------------------------------
-// K loop of the 8-wave, 256-query tile fed by global -> LDS loads into a ring
-// of kRing stage buffers (kRing - 1 stages ahead).  No staging registers and
-// no LDS store instructions: each wave loads 64 of the stage's 512 rows
-// (waves 0-3 index rows, 4-7 query rows) as 4 x 16-row pieces of 1 KB, lane
-// l writing LDS bytes l*16 of its piece; the lane fetches the logical chunk
-// that the 64-byte swizzle puts there.  A stage waits for its own loads
-// (vmcnt counts this wave's 4 loads per stage) before the barrier that
-// publishes them, and reuses the buffer of the stage before it.
+// K loop of the 8-wave ring tiles (256, 128 or 64 queries) fed by global ->
+// LDS loads into a ring of kRing stage buffers (kRing - 1 stages ahead).  No
+// staging registers and no LDS store instructions: waves 0-3 load the
+// stage's 256 index rows (64 each, 4 x 16-row pieces of 1 KB), waves 4-7 its
+// BR query rows (BR / 4 each, BR / 64 pieces); lane l writes LDS bytes l*16
+// of its piece and fetches the logical chunk that the 64-byte swizzle puts
+// there.  A stage waits for its own loads (vmcnt counts this wave's loads
+// per stage) before the barrier that publishes them, and reuses the buffer
+// of the stage before it.
 // ---------------------------------------------------------------------------
 static_assert(!kRingOk || kSwz == 1, "ring rows use the 64-byte swizzle");
 
@@ -250,101 +274,128 @@ struct RingSrc {
     int voff;                  // lane offset of its first 16-row piece
     int step16;                // bytes between pieces (16 rows)
     int lds_off;               // LDS offset of the wave's rows in a stage buffer
+    int wid;                   // wave id (uniform)
 };
 
+// loads per stage of an index-row wave (64 rows in 16-row pieces)
+constexpr int kRingLoads = 4;
+static_assert((kRing - 2) * kRingLoads == 8, "ring_wait's index waves wait with vmcnt(8)");
+
+template <class SH>
 __device__ __forceinline__ RingSrc ring_src(const TileSrc &ts, int tid) {
+    static_assert(SH::NT == 512 && SH::BR % 64 == 0, "ring tiles: 8 waves, 16-row query pieces");
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: no waterfall
     const int lane = tid & 63;
-    // LDS row (wid*64 + piece*16 + lane/4) has ((row >> 2) & 3) = (lane >> 4) & 3
+    // LDS row (16-aligned piece base + lane/4) has ((row >> 2) & 3) = (lane >> 4) & 3
     const int c = (lane & 3) ^ ((lane >> 4) & 3);
-    const int ld = wid < 4 ? ts.ldb : ts.ldq;
+    constexpr int QR = SH::BR / 4;  // query rows per query wave
+    const bool iw = wid < 4;         // an index-row wave
+    const int ld = iw ? ts.ldb : ts.ldq;
     RingSrc d;
-    d.r = wid < 4 ? ts.db : ts.q;
-    d.voff = ((wid & 3) * 64 + (lane >> 2)) * ld + c * 16;
+    d.wid = wid;
+    d.r = iw ? ts.db : ts.q;
+    d.voff = ((wid & 3) * (iw ? 64 : QR) + (lane >> 2)) * ld + c * 16;
     d.step16 = 16 * ld;
-    d.lds_off = wid * 64 * kRow;
+    d.lds_off = (iw ? wid * 64 : kBI + (wid & 3) * QR) * kRow;
     return d;
 }
 
-// loads per wave per stage: 512 rows over 8 waves, 16 rows per load
-constexpr int kRingLoads = Tile<4, 2>::ROWS / 8 / 16;
-static_assert(kRingLoads == 4, "the ring's vmcnt waits count 4 loads per stage");
-static_assert((kRing - 2) * kRingLoads == 8, "ring_stage waits with vmcnt(8)");
-
+template <class SH>
 __device__ __forceinline__ void ring_load(const RingSrc &d, uint8_t *sb, int kb) {
+    constexpr int QL = SH::BR / 64;  // loads per stage of a query wave
 #pragma unroll
     for (int j = 0; j < kRingLoads; ++j)
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            d.r, (__attribute__((address_space(3))) void *)(sb + d.lds_off + j * 16 * kRow), 16,
-            d.voff, kb + j * d.step16, 0, 0);
+        if (j < QL || d.wid < 4)  // wave-uniform
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                d.r, (__attribute__((address_space(3))) void *)(sb + d.lds_off + j * 16 * kRow), 16,
+                d.voff, kb + j * d.step16, 0, 0);
+}
+
+// Wait until the stage issued kRing - 1 stages ago has landed in this wave's
+// rows (two younger stages stay in flight: vmcnt = 2 x the wave's loads per
+// stage), then the barrier that publishes it.  Written out so that the wait
+// stays at that count instead of the vmcnt(0) a fence would add.
+template <class SH>
+__device__ __forceinline__ void ring_wait(int wid) {
+    static_assert(kRing == 4, "the waits below assume two younger stages in flight");
+    if constexpr (SH::BR == 256) {
+        asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    } else if (wid < 4) {
+        asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    } else if constexpr (SH::BR == 128) {
+        asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    } else {
+        static_assert(SH::BR == 64, "ring tiles of 256, 128 or 64 queries");
+        asm volatile("s_waitcnt vmcnt(2) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    }
 }
 
 // One stage: wait until stage s (issued kRing - 1 stages ago) has landed in
 // every wave's rows, then load stage s + kRing - 1 into the buffer stage s - 1
 // left and run stage s's MFMAs.  The restrict pointers give the fragment
 // reads an alias scope apart from the loads, so the compiler does not wait
-// for the loads in flight before each read; the barrier is written out so
-// that its wait stays vmcnt(8) (two younger stages in flight) instead of the
-// vmcnt(0) a fence would add.
-template <int ELEM, bool FIRST>
-__device__ __forceinline__ void ring_stage(v16f (&acc)[4][2], const RingSrc &d,
+// for the loads in flight before each read.
+template <int ELEM, class SH, bool FIRST>
+__device__ __forceinline__ void ring_stage(v16f (&acc)[SH::AI][2], const RingSrc &d,
                                            uint8_t *__restrict__ bw, const uint8_t *__restrict__ br,
                                            int s, int nk, int wi, int wr, int lane) {
-    asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");  // (kRing-2)*kRingLoads
-    Frags<2> f;
-    ld_frags_b<ELEM, 2>(f, br, wr, lane);
-    ld_frags_a<ELEM, 2, 0, 2>(f, br, wi, lane);
-    ring_load(d, bw, min(s + kRing - 1, nk - 1) * kBK);  // past the end: an unread buffer
-    mma_part<ELEM, 2, 0, 2, FIRST>(acc, f);
-    ld_frags_a<ELEM, 2, 2, 4>(f, br, wi, lane);
-    mma_part<ELEM, 2, 2, 4, FIRST>(acc, f);
+    constexpr int AI = SH::AI, H = (AI + 1) / 2;
+    ring_wait<SH>(d.wid);
+    Frags<AI> f;
+    ld_frags_b<ELEM, AI>(f, br, wr, lane);
+    ld_frags_a<ELEM, AI, 0, H>(f, br, wi, lane);
+    ring_load<SH>(d, bw, min(s + kRing - 1, nk - 1) * kBK);  // past the end: an unread buffer
+    mma_part<ELEM, AI, 0, H, FIRST>(acc, f);
+    ld_frags_a<ELEM, AI, H, AI>(f, br, wi, lane);
+    mma_part<ELEM, AI, H, AI, FIRST>(acc, f);
 }
 
 // A stage of the streamed loop that loads stage `kb` of `d` (the next tile)
 // into bw, or (LOAD false) loads nothing.
-template <int ELEM, bool LOAD>
-__device__ __forceinline__ void ring_stage_next(v16f (&acc)[4][2], const RingSrc &d,
+template <int ELEM, class SH, bool LOAD>
+__device__ __forceinline__ void ring_stage_next(v16f (&acc)[SH::AI][2], const RingSrc &d,
                                                 uint8_t *__restrict__ bw,
                                                 const uint8_t *__restrict__ br, int kb, int wi,
                                                 int wr, int lane) {
-    asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    Frags<2> f;
-    ld_frags_b<ELEM, 2>(f, br, wr, lane);
-    ld_frags_a<ELEM, 2, 0, 2>(f, br, wi, lane);
-    if constexpr (LOAD) ring_load(d, bw, kb);
-    mma_part<ELEM, 2, 0, 2, false>(acc, f);
-    ld_frags_a<ELEM, 2, 2, 4>(f, br, wi, lane);
-    mma_part<ELEM, 2, 2, 4, false>(acc, f);
+    constexpr int AI = SH::AI, H = (AI + 1) / 2;
+    ring_wait<SH>(d.wid);
+    Frags<AI> f;
+    ld_frags_b<ELEM, AI>(f, br, wr, lane);
+    ld_frags_a<ELEM, AI, 0, H>(f, br, wi, lane);
+    if constexpr (LOAD) ring_load<SH>(d, bw, kb);
+    mma_part<ELEM, AI, 0, H, false>(acc, f);
+    ld_frags_a<ELEM, AI, H, AI>(f, br, wi, lane);
+    mma_part<ELEM, AI, H, AI, false>(acc, f);
 }
 
-template <int ELEM>
-__device__ __forceinline__ void tile_product_ring(v16f (&acc)[4][2], const GemmArgs &a, int64_t i0,
-                                                  int64_t r0, uint8_t *ring, int tid, int wi,
-                                                  int wr, int lane) {
+template <int ELEM, class SH>
+__device__ __forceinline__ void tile_product_ring(v16f (&acc)[SH::AI][2], const GemmArgs &a,
+                                                  int64_t i0, int64_t r0, uint8_t *ring, int tid,
+                                                  int wi, int wr, int lane) {
     TileSrc ts;
     ts.db = rows_rsrc(a.db, i0, a.N, kBI, a.ldb);
-    ts.q = rows_rsrc(a.q, r0, a.M, Tile<4, 2>::BR, a.ldq);
+    ts.q = rows_rsrc(a.q, r0, a.M, SH::BR, a.ldq);
     ts.ldb = (int)a.ldb;
     ts.ldq = (int)a.ldq;
-    const RingSrc d = ring_src(ts, tid);
+    const RingSrc d = ring_src<SH>(ts, tid);
     const int nk = a.Kb / kBK;  // >= 1
-    constexpr int SB = Tile<4, 2>::STAGE_BYTES;
+    constexpr int SB = SH::STAGE_BYTES;
     uint8_t *b0 = ring, *b1 = ring + SB, *b2 = ring + 2 * SB, *b3 = ring + 3 * SB;
     // the callers' last barrier ordered every earlier use of the ring
-    ring_load(d, b0, 0);
-    ring_load(d, b1, min(1, nk - 1) * kBK);
-    ring_load(d, b2, min(2, nk - 1) * kBK);
-    ring_stage<ELEM, true>(acc, d, b3, b0, 0, nk, wi, wr, lane);
+    ring_load<SH>(d, b0, 0);
+    ring_load<SH>(d, b1, min(1, nk - 1) * kBK);
+    ring_load<SH>(d, b2, min(2, nk - 1) * kBK);
+    ring_stage<ELEM, SH, true>(acc, d, b3, b0, 0, nk, wi, wr, lane);
     int s = 1;
     for (; s + 3 < nk; s += 4) {
-        ring_stage<ELEM, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
-        ring_stage<ELEM, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
-        ring_stage<ELEM, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
-        ring_stage<ELEM, false>(acc, d, b3, b0, s + 3, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, false>(acc, d, b3, b0, s + 3, nk, wi, wr, lane);
     }
-    if (s < nk) ring_stage<ELEM, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
-    if (s + 1 < nk) ring_stage<ELEM, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
-    if (s + 2 < nk) ring_stage<ELEM, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
+    if (s < nk) ring_stage<ELEM, SH, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
+    if (s + 1 < nk) ring_stage<ELEM, SH, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
+    if (s + 2 < nk) ring_stage<ELEM, SH, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
     // every load of the tile (the clamped ones too) has landed and every
     // fragment read is done before the epilogue reuses the ring
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
@@ -358,48 +409,48 @@ __device__ __forceinline__ void tile_product_ring(v16f (&acc)[4][2], const GemmA
 // epilogue's scratch).  The last tile of an item loads its last stage again
 // in place of the next tile's (into buffers 0-1, which the next item's
 // first tile refills after the item's closing barrier) so that every stage
-// keeps two younger stages in flight for the vmcnt(8) wait.
-template <int ELEM>
-__device__ __forceinline__ void tile_product_ring_stream(v16f (&acc)[4][2], const GemmArgs &a,
+// keeps two younger stages in flight for ring_wait.
+template <int ELEM, class SH>
+__device__ __forceinline__ void tile_product_ring_stream(v16f (&acc)[SH::AI][2], const GemmArgs &a,
                                                          int64_t i0, int64_t r0, uint8_t *ring,
                                                          int tid, int wi, int wr, int lane,
                                                          bool first, bool has_next) {
     TileSrc ts;
     ts.db = rows_rsrc(a.db, i0, a.N, kBI, a.ldb);
-    ts.q = rows_rsrc(a.q, r0, a.M, Tile<4, 2>::BR, a.ldq);
+    ts.q = rows_rsrc(a.q, r0, a.M, SH::BR, a.ldq);
     ts.ldb = (int)a.ldb;
     ts.ldq = (int)a.ldq;
-    const RingSrc d = ring_src(ts, tid);
+    const RingSrc d = ring_src<SH>(ts, tid);
     const int nk = a.Kb / kBK;  // a multiple of 4
     RingSrc dn = d;             // next tile (or this one again)
     int kn = (nk - 1) * kBK;    // next tile's stage 0 (or this tile's last)
     if (has_next) {
         ts.db = rows_rsrc(a.db, i0 + kBI, a.N, kBI, a.ldb);
-        dn = ring_src(ts, tid);
+        dn = ring_src<SH>(ts, tid);
         kn = 0;
     }
-    constexpr int SB = Tile<4, 2>::STAGE_BYTES;
+    constexpr int SB = SH::STAGE_BYTES;
     uint8_t *b0 = ring, *b1 = ring + SB, *b2 = ring + 2 * SB, *b3 = ring + 3 * SB;
     // the callers' last barrier ordered every earlier use of buffers 2-3
     // (and of 0-1 on an item's first tile; this wave's loads into its rows of
     // 0-1 from the previous item have landed)
     if (first) {
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        ring_load(d, b0, 0);
-        ring_load(d, b1, kBK);
+        ring_load<SH>(d, b0, 0);
+        ring_load<SH>(d, b1, kBK);
     }
-    ring_load(d, b2, 2 * kBK);
-    ring_stage<ELEM, true>(acc, d, b3, b0, 0, nk, wi, wr, lane);
+    ring_load<SH>(d, b2, 2 * kBK);
+    ring_stage<ELEM, SH, true>(acc, d, b3, b0, 0, nk, wi, wr, lane);
     for (int s = 1; s + 3 < nk; s += 4) {
-        ring_stage<ELEM, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
-        ring_stage<ELEM, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
-        ring_stage<ELEM, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
-        ring_stage<ELEM, false>(acc, d, b3, b0, s + 3, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, false>(acc, d, b3, b0, s + 3, nk, wi, wr, lane);
     }
     // stages nk-3 .. nk-1 (buffers 1, 2, 3) load the next tile's 0 and 1
-    ring_stage_next<ELEM, true>(acc, dn, b0, b1, kn, wi, wr, lane);
-    ring_stage_next<ELEM, true>(acc, dn, b1, b2, kn + (has_next ? kBK : 0), wi, wr, lane);
-    ring_stage_next<ELEM, false>(acc, dn, b2, b3, 0, wi, wr, lane);
+    ring_stage_next<ELEM, SH, true>(acc, dn, b0, b1, kn, wi, wr, lane);
+    ring_stage_next<ELEM, SH, true>(acc, dn, b1, b2, kn + (has_next ? kBK : 0), wi, wr, lane);
+    ring_stage_next<ELEM, SH, false>(acc, dn, b2, b3, 0, wi, wr, lane);
     // every fragment read is done before the epilogue uses buffers 2-3
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
@@ -427,10 +478,11 @@ __device__ __forceinline__ bool block_any(bool p, int *flags, int &k) {
 }
 
 // Accumulator i of block (ai, bj) at lane l: index row
-// wi*128 + ai*32 + 8*(i/4) + 4*(l/32) + (i%4), query wr*32*BB + bj*32 + l%32
+// wi*32*AI + ai*32 + 8*(i/4) + 4*(l/32) + (i%4), query wr*64 + bj*32 + l%32
 // (the 32x32 MFMA output layout).
+template <int AI>
 __device__ __forceinline__ int acc_row(int wi, int ai, int i, int lane) {
-    return wi * 128 + ai * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+    return wi * (32 * AI) + ai * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
 }
 
 // Decode a workgroup's (row tile, split) from the launch.  Dispatch is
@@ -450,26 +502,28 @@ __device__ __forceinline__ bool decode_item(const GemmArgs &a, int64_t g, int64_
     return grp * per < (int64_t)a.ngroups * per && rt < a.R;
 }
 
-template <int ELEM, int WR, int BB>
+template <int ELEM, class SH>
 __device__ __forceinline__ void store_item(const GemmArgs &a, int64_t rt, int split) {
     extern __shared__ __align__(16) uint8_t smem[];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid & 1, wr = wid >> 1;
-    uint8_t *sb0 = smem, *sb1 = smem + Tile<WR, BB>::STAGE_BYTES;
-    const int64_t r0 = rt * Tile<WR, BB>::BR;
+    constexpr int AI = SH::AI;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid % SH::WI,
+              wr = wid / SH::WI;
+    uint8_t *sb0 = smem, *sb1 = smem + SH::STAGE_BYTES;
+    const int64_t r0 = rt * SH::BR;
     const int64_t t0 = (int64_t)split * a.tiles / a.S, t1 = (int64_t)(split + 1) * a.tiles / a.S;
-    v16f acc[4][BB];
+    v16f acc[AI][2];
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
-        tile_product<ELEM, WR, BB>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
+        tile_product<ELEM, SH>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
 #pragma unroll
-        for (int bj = 0; bj < BB; ++bj) {
-            const int64_t q = r0 + wr * 32 * BB + bj * 32 + (lane & 31);
+        for (int bj = 0; bj < 2; ++bj) {
+            const int64_t q = r0 + wr * 64 + bj * 32 + (lane & 31);
             if (q >= a.M) continue;
 #pragma unroll
-            for (int ai = 0; ai < 4; ++ai)
+            for (int ai = 0; ai < AI; ++ai)
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    const int64_t col = i0 + acc_row(wi, ai, i, lane);
+                    const int64_t col = i0 + acc_row<AI>(wi, ai, i, lane);
                     if (col < a.N) {
                         const int64_t o = q + col * a.ldc;
                         if (a.c_dtype == RM_F32) ((float *)a.C)[o] = acc[ai][bj][i];
@@ -485,13 +539,13 @@ __device__ __forceinline__ void store_item(const GemmArgs &a, int64_t rt, int sp
 // grid of one workgroup per item runs each item once; a persistent grid
 // (a multiple of 8 workgroups, at most one per CU) keeps every workgroup on
 // its XCD's item range, since the stride preserves it % 8.
-template <int ELEM, int WR, int BB>
-__global__ __launch_bounds__(128 * WR, 1) void gemm_store_kernel(GemmArgs a) {
+template <int ELEM, class SH>
+__global__ __launch_bounds__(SH::NT, 1) void gemm_store_kernel(GemmArgs a) {
     for (int64_t it = blockIdx.x; it < a.items8; it += gridDim.x) {
         int64_t rt;
         int split;
         if (!decode_item(a, it, a.items8, rt, split)) continue;
-        store_item<ELEM, WR, BB>(a, rt, split);
+        store_item<ELEM, SH>(a, rt, split);
     }
 }
 
@@ -682,20 +736,22 @@ __device__ __forceinline__ void load_floor(const GemmArgs &a, float *floorv, int
     }
 }
 
-template <int ELEM, int WR, int BB>
+template <int ELEM, class SH>
 __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int split) {
-    constexpr int BR = Tile<WR, BB>::BR;
-    constexpr int NT = Tile<WR, BB>::NT;
+    constexpr int BR = SH::BR;
+    constexpr int NT = SH::NT;
+    constexpr int AI = SH::AI, BB = SH::BB;
     extern __shared__ __align__(16) uint8_t smem[];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid & 1, wr = wid >> 1;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid % SH::WI,
+              wr = wid / SH::WI;
     const int KP = a.KP, LP = KP + 1, capd = a.capd, CP = capd + 1;
-    static_assert(2 * Tile<WR, BB>::STAGE_BYTES == stage_pair_bytes(BR), "stage layout");
-    // RING: the 8-wave tile streams K through the ring and keeps the lists in
+    static_assert(2 * SH::STAGE_BYTES == stage_pair_bytes(BR), "stage layout");
+    // RING: the 8-wave tiles stream K through the ring and keep the lists in
     // the split partials; the epilogue's scratch (first-tile seed, lists
     // staged for a merge: BR x 32 values in ring buffer 2, columns in 3) sits
     // in the ring buffers no load is in flight to
-    constexpr bool RING = kRingOk && WR == 4 && BB == 2;
-    constexpr int SB = Tile<WR, BB>::STAGE_BYTES;
+    constexpr bool RING = kRingOk && SH::RING;
+    constexpr int SB = SH::STAGE_BYTES;
     static_assert(!RING || kRing * SB == ring_bytes(BR), "ring layout");
     static_assert(!RING || BR * 32 * 4 <= SB, "staged lists fit one ring buffer each");
     uint8_t *sb0 = smem, *sb1 = smem + SB;
@@ -737,20 +793,20 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
         qv[bj] = r0 + ql[bj] < a.M;
     }
     // accumulator (ai, i) of this lane sits at tile column rowbase + RC(ai, i)
-    const int rowbase = wi * 128 + 4 * (lane >> 5);
+    const int rowbase = wi * (32 * AI) + 4 * (lane >> 5);
 #define RC(ai, i) ((ai) * 32 + 8 * ((i) >> 2) + ((i) & 3))
-    v16f acc[4][BB];
+    v16f acc[AI][BB];
     uint8_t *const ring = smem + (RING ? opaque_uniform(0) : 0);
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
         if constexpr (RING) {
             if ((a.Kb / kBK) % 4 == 0)
-                tile_product_ring_stream<ELEM>(acc, a, i0, r0, ring, tid, wi, wr, lane, t == t0,
-                                               t + 1 < t1);
+                tile_product_ring_stream<ELEM, SH>(acc, a, i0, r0, ring, tid, wi, wr, lane, t == t0,
+                                                   t + 1 < t1);
             else
-                tile_product_ring<ELEM>(acc, a, i0, r0, ring, tid, wi, wr, lane);
+                tile_product_ring<ELEM, SH>(acc, a, i0, r0, ring, tid, wi, wr, lane);
         } else
-            tile_product<ELEM, WR, BB>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
+            tile_product<ELEM, SH>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
         // columns of this tile valid for this lane: RC(ai, i) < lim
         const int lim = (int)min<int64_t>(a.N - i0 - rowbase, (int64_t)kBI);
         const int32_t colbase = (int32_t)(a.col_base + i0 + rowbase);
@@ -759,7 +815,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
             // columns past N (rows read as zeros) score -inf: never above tau,
             // and they can only lower the first-tile bound
 #pragma unroll
-            for (int ai = 0; ai < 4; ++ai)
+            for (int ai = 0; ai < AI; ++ai)
 #pragma unroll
                 for (int bj = 0; bj < BB; ++bj)
 #pragma unroll
@@ -769,18 +825,19 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
 
         // --- first tile of the item: seed tau with a lower bound of each
         // query's k'-th value, so the first tiles do not append every value.
-        // A lane's 64 values of a query form 4 groups of 16 (ai); a group's
-        // second largest has 2 values at or above it, so over the query's 4
-        // lanes (2 per wave, 2 waves) the smallest second maximum has >= 32
-        // values at or above it (the smallest maximum: >= 16).  Values equal
-        // to the bound must stay candidates: tau is the next float below.
+        // A lane's values of a query form AI groups of 16 (ai), and a query
+        // has 16 such groups over its 2 * WI lanes (2 per wave); a group's
+        // second largest has 2 values at or above it, so the smallest second
+        // maximum has >= 32 values at or above it (the smallest maximum:
+        // >= 16).  Values equal to the bound must stay candidates: tau is the
+        // next float below.
         if (t == t0 && KP <= 32) {
             int key[BB];
 #pragma unroll
             for (int bj = 0; bj < BB; ++bj) {
                 float bound = INFINITY;
 #pragma unroll
-                for (int ai = 0; ai < 4; ++ai) {
+                for (int ai = 0; ai < AI; ++ai) {
                     float m1 = -INFINITY, m2 = -INFINITY;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
@@ -819,13 +876,13 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
             if (share) tau = fmaxf(tau, floorv[ql[bj]]);  // floor: strictly above = at or above the bound
             float mx = -INFINITY;
 #pragma unroll
-            for (int ai = 0; ai < 4; ++ai)
+            for (int ai = 0; ai < AI; ++ai)
 #pragma unroll
                 for (int i = 0; i < 16; i += 2)
                     mx = fmaxf(mx, fmaxf(acc[ai][bj][i], acc[ai][bj][i + 1]));
             if (qv[bj] && mx > tau) {
 #pragma unroll
-                for (int ai = 0; ai < 4; ++ai)
+                for (int ai = 0; ai < AI; ++ai)
 #pragma unroll
                     for (int i = 0; i < 16; ++i)
                         if (acc[ai][bj][i] > tau) pend[bj] |= 1ull << (ai * 16 + i);
@@ -846,7 +903,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                 int slot = atomicAdd(&cnt[q], __popcll(pend[bj]));
                 if (slot >= capd) continue;
 #pragma unroll
-                for (int ai = 0; ai < 4; ++ai) {
+                for (int ai = 0; ai < AI; ++ai) {
                     if (!((pend[bj] >> (ai * 16)) & 0xffffull)) continue;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
@@ -884,7 +941,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                 const float tau = tauv[ql[bj]];
                 const float fl = share ? floorv[ql[bj]] : -INFINITY;
 #pragma unroll
-                for (int ai = 0; ai < 4; ++ai) {
+                for (int ai = 0; ai < AI; ++ai) {
                     if (!((pend[bj] >> (ai * 16)) & 0xffffull)) continue;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
@@ -921,13 +978,13 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     store_lists<BR, NT>(lv, lc, a, r0, split, KP, at, tid);
 }
 
-template <int ELEM, int WR, int BB>
-__global__ __launch_bounds__(128 * WR, 1) void gemm_topk_kernel(GemmArgs a) {
+template <int ELEM, class SH>
+__global__ __launch_bounds__(SH::NT, 1) void gemm_topk_kernel(GemmArgs a) {
     for (int64_t it = blockIdx.x; it < a.items8; it += gridDim.x) {
         int64_t rt;
         int split;
         if (!decode_item(a, it, a.items8, rt, split)) continue;
-        topk_item<ELEM, WR, BB>(a, rt, split);
+        topk_item<ELEM, SH>(a, rt, split);
         __syncthreads();  // the next item re-initialises LDS (and refills the ring)
     }
 }
@@ -947,9 +1004,9 @@ static int launch_common(const GemmArgs &a0, const void *fn, size_t lds, int nt,
     return RM_OK;
 }
 
-// Kernel shapes: 8 waves (two per SIMD) with 128 x 64 wave tiles over
-// 256 x 256 tiles, fed by the load ring; lists longer than 32 use 4 waves x
-// 128 x 64 (128 queries per tile) with register staging.
+// Kernel shapes: 8 waves (two per SIMD) over tiles of 256 index rows x 256,
+// 128 or 64 queries, fed by the load ring; lists longer than 32 use 4 waves
+// x 128 x 64 (128 queries per tile) with register staging.
 
 // order key of -inf: the shared bounds' starting value
 constexpr int32_t kNegInfKey = (int32_t)(0xff800000u ^ 0x7fffffffu);
@@ -960,9 +1017,29 @@ __global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
         p[i] = v;
 }
 
+template <class SH>
+static const void *topk_fn(int elem) {
+    return elem == 0 ? (const void *)gemm_topk_kernel<0, SH> : (const void *)gemm_topk_kernel<1, SH>;
+}
+
+// k' <= 32: the ring kernel of bq queries per tile (with the unswizzled
+// LDS layouts: the register-staged kernel of 256)
+template <bool RING_OK>
+static const void *short_list_fn(int bq, int elem) {
+    if constexpr (RING_OK)
+        return bq == 256 ? topk_fn<ShapeTopk256>(elem)
+               : bq == 128 ? topk_fn<ShapeTopk128>(elem) : topk_fn<ShapeTopk64>(elem);
+    else
+        return topk_fn<Shape<512, 256, false>>(elem);
+}
+
 int launch_gemm_topk(const GemmArgs &a0, int elem, hipStream_t st) {
     GemmArgs a = a0;
-    a.capd = topk_cand_slots(a.KP);
+    RM_CHECK(tile_rows_ok(a.KP, a.BQ), RM_ERR_ARG, "no fused kernel for %d queries per tile at k' = %d",
+             a.BQ, a.KP);
+    RM_CHECK(a.R == cdiv(a.M, a.BQ), RM_ERR_ARG, "row tiles %lld != ceil(M / %d)", (long long)a.R,
+             a.BQ);
+    a.capd = topk_cand_slots(a.KP, a.BQ);
     a.bound = nullptr;
     if (uses_ring(a.KP) && a.S > 1 && a.M > 0) {
         // the bounds follow the partials (part_bytes holds M more entries);
@@ -973,19 +1050,21 @@ int launch_gemm_topk(const GemmArgs &a0, int elem, hipStream_t st) {
     }
     RM_CHECK(a.capd >= 1, RM_ERR_SHAPE, "no LDS left for candidate slots (k' = %d)", a.KP);
     const void *fn;
-    int nt = 256;
-    if (rows_per_tile(a.KP) == 128)
-        fn = elem == 0 ? (const void *)gemm_topk_kernel<0, 2, 2> : (const void *)gemm_topk_kernel<1, 2, 2>;
-    else {
-        fn = elem == 0 ? (const void *)gemm_topk_kernel<0, 4, 2> : (const void *)gemm_topk_kernel<1, 4, 2>;
-        nt = 512;
+    int nt = 512;
+    if (!ring_list(a.KP)) {
+        fn = topk_fn<ShapeList128>(elem);
+        nt = ShapeList128::NT;
+    } else {
+        RM_CHECK(kRingOk || a.BQ == 256, RM_ERR_ARG, "unswizzled LDS layouts run 256 queries per tile");
+        fn = short_list_fn<kRingOk>(a.BQ, elem);
     }
-    return launch_common(a, fn, topk_lds_bytes(a.KP), nt, st);
+    return launch_common(a, fn, topk_lds_bytes(a.KP, a.BQ), nt, st);
 }
 int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st) {
     const void *fn;
-    fn = elem == 0 ? (const void *)gemm_store_kernel<0, 4, 2> : (const void *)gemm_store_kernel<1, 4, 2>;
-    return launch_common(a, fn, store_lds_bytes(), 512, st);
+    fn = elem == 0 ? (const void *)gemm_store_kernel<0, ShapeStore>
+                   : (const void *)gemm_store_kernel<1, ShapeStore>;
+    return launch_common(a, fn, store_lds_bytes(), ShapeStore::NT, st);
 }
 
 // ---------------------------------------------------------------------------

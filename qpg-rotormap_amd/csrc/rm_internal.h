@@ -55,9 +55,14 @@ constexpr int kRow = kSwz ? 64 : 80;  // LDS row pitch in bytes
 constexpr int kThreads = 256;  // 4 waves
 constexpr int kMaxKP = 64;     // largest top-k' list
 
-// Queries per tile of the fused kernel: 256 while the per-query lists fit
-// LDS beside the stage buffers (k' <= 32), else 128.
-inline int rows_per_tile(int kp) { return kp <= 32 ? 256 : 128; }
+// Queries per tile of the fused kernel.  Lists of k' <= 32 run on the ring
+// kernels with 256, 128 or 64 queries per tile (the tile plan picks one: a
+// small batch wastes less MFMA work on padded query rows with a smaller
+// tile); longer lists use the register-staged kernel with 128.
+inline bool ring_list(int kp) { return kp <= 32; }
+inline bool tile_rows_ok(int kp, int bq) {
+    return ring_list(kp) ? (bq == 256 || bq == 128 || bq == 64) : bq == 128;
+}
 // query bytes the fused kernel's concurrent workgroups may re-read from the
 // MALL (half of MI355X's 256 MB Infinity Cache)
 constexpr int64_t kQueryWorkingSet = 128LL << 20;
@@ -90,7 +95,7 @@ constexpr size_t topk_lds_bytes_for(int br, int kp) {
 // buffer 3, and the first tile's bound seed uses buffer 2.
 constexpr int kRing = 4;
 constexpr bool kRingOk = kSwz == 1;
-inline bool uses_ring(int kp) { return kRingOk && rows_per_tile(kp) == 256; }
+inline bool uses_ring(int kp) { return kRingOk && ring_list(kp); }
 constexpr size_t ring_bytes(int br) { return (size_t)kRing * (kBI + br) * kRow; }
 constexpr size_t ring_cand_offset(int br) { return ring_bytes(br); }
 // the ring layout's tail: tau, counters, flags, then the per-query floor
@@ -107,13 +112,23 @@ constexpr size_t ring_lds_bytes(int br) {
 constexpr size_t ring_buffer_bytes(int br) { return (size_t)(kBI + br) * kRow; }
 // one array (values or columns) of the lists staged for a merge
 constexpr size_t ring_staged_list_bytes(int br) { return (size_t)br * 32 * 4; }
-inline int topk_cand_slots(int kp) {
-    return uses_ring(kp) ? ring_cand_slots(256) : cand_slots(rows_per_tile(kp), kp);
+// bq: queries per tile (tile_rows_ok); without the ring (k' > 32, or the
+// unswizzled layouts) the register-staged kernel's tile of 128 or 256
+inline int topk_cand_slots(int kp, int bq) {
+    return uses_ring(kp) ? ring_cand_slots(bq) : cand_slots(ring_list(kp) ? 256 : 128, kp);
 }
-inline size_t topk_lds_bytes(int kp) {
-    return uses_ring(kp) ? ring_lds_bytes(256) : topk_lds_bytes_for(rows_per_tile(kp), kp);
+inline size_t topk_lds_bytes(int kp, int bq) {
+    return uses_ring(kp) ? ring_lds_bytes(bq) : topk_lds_bytes_for(ring_list(kp) ? 256 : 128, kp);
 }
 inline size_t store_lds_bytes() { return 2 * (size_t)(kBI + 256) * kRow; }
+
+// Tile plan of one fused launch: query rows per tile, column splits and the
+// query tiles grouped per XCD (0 = all).
+struct Plan {
+    int bq = 256;
+    int S = 1;
+    int qg = 0;
+};
 
 struct GemmArgs {
     const uint8_t *db;  // index rows (K contiguous bytes each), row pitch ldb bytes
@@ -121,6 +136,7 @@ struct GemmArgs {
     const uint8_t *q;   // query rows, row pitch ldq bytes
     int64_t ldq;
     int64_t M;          // query rows
+    int BQ;             // query rows per tile (tile_rows_ok)
     int64_t N;          // index rows covered by this launch
     int Kb;             // K in bytes
     int S;              // column splits
@@ -200,8 +216,8 @@ struct rm_ctx {
     std::map<std::tuple<int, int, int>, int32_t *> tables;  // (k, s, m) -> device phasor table
     size_t table_bytes = 0;
     std::map<std::string, rm_buf> ws;
-    // autotuned splits: (mode, M bucket) -> S
-    std::map<std::pair<int, int>, std::pair<int, int>> plan;  // -> (splits, qgroup)
+    // tile plans: (mode, M bucket ceil(log2 M)) -> plan (rm_autotune, rm_plan_set)
+    std::map<std::pair<int, int>, rm::Plan> plan;
     // kernel timing (rm_prof_enable): event pairs around fused launches
     bool persistent = true;  // fused launches use a grid of one workgroup per CU
     bool prof = false;
@@ -221,23 +237,26 @@ namespace rm {
 int ws_get(rm_ctx *ctx, const char *name, size_t bytes, void **out);
 int table_get(rm_ctx *ctx, int k, int s, int m, const int32_t **out);
 hipStream_t pick_stream(rm_ctx *ctx, void *stream);
-int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int kp, int64_t row_bytes);
+int default_splits(rm_ctx *ctx, int64_t M, int64_t N, int bq, int64_t row_bytes);
 // the whole fused query path on already-encoded queries
 int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
               const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
-              float *vals, int32_t *locs, int64_t ldo, int S, int qg, hipStream_t st);
+              float *vals, int32_t *locs, int64_t ldo, const Plan &pl, hipStream_t st);
 // the same with caller-owned split partials (pv, pc of pbytes each)
 int topk_core_bufs(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
                    const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
-                   float *vals, int32_t *locs, int64_t ldo, int S, int qg, float *pv, int32_t *pc,
-                   size_t pbytes, hipStream_t st);
+                   float *vals, int32_t *locs, int64_t ldo, const Plan &pl, float *pv,
+                   int32_t *pc, size_t pbytes, hipStream_t st);
 // bytes of each split-partial array for M rows, list length KP, S splits
 size_t part_bytes(const rm_index *idx, int64_t M, int KP, int S);
+// the largest split count <= S whose partials for M rows fit pbytes (>= 1)
+int max_splits_within(const rm_index *idx, int64_t M, int KP, int S, size_t pbytes);
 // encode n fragments into caller-owned query buffers (q32 only for rescoring)
 int encode_into(rm_ctx *ctx, const uint32_t *words, int64_t n, int k, int s, int m, int c,
                 int mode, int64_t rdim, float *norms, void *q, float *q32, hipStream_t st);
 int check_encoder(int k, int s, int m, int c, int normalize);
 int check_query(const rm_index *idx, int ktop, int kprime, int mode);
-// (splits, query tiles per XCD group) for a batch of M rows
-std::pair<int, int> plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp);
+// the tile plan of a batch of M rows (kp: the list length the fused kernel
+// keeps, kprime for RM_MODE_FP8_RESCORE and ktop otherwise)
+Plan plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp);
 }  // namespace rm

@@ -40,7 +40,6 @@ struct Req {
     int64_t next_row = 0;            // first row not yet placed in a launch
     int64_t rows_done = 0;
     int pending = 0;                 // pieces of this request in launch slots
-    bool waited = false;             // failed and already reported to a waiter
     int state = Q_QUEUED;
     double t_sub = 0, t_adm = -1, t_fin = -1;
     std::vector<float> vals;         // n x ktop row-major
@@ -88,6 +87,7 @@ struct rm_server {
     int64_t next_id = 0;
     int policy = RM_SCHED_FIFO;
     bool stop = false;
+    bool disp_done = false;  // the dispatcher has exited: no launch will join inflight
     int64_t launches = 0, rows_done = 0, max_inflight = 0;
     clk::time_point t0 = clk::now();
     std::thread disp, comp;
@@ -99,15 +99,18 @@ namespace {
 using namespace rm;
 
 // device buffers of one slot; part bytes cover the largest split plan any
-// batch of <= max_rows rows gets
+// batch of <= max_rows rows gets from the plans of the server's creation
+// (launch_slot lowers the split count of a plan set later to what fits)
 int64_t slot_bytes(rm_server *sv, size_t *pbytes_out) {
     const int KP = sv->mode == RM_MODE_FP8_RESCORE ? sv->kprime : sv->ktop;
     size_t pb = 0;
-    const int br = rows_per_tile(KP);
-    for (int64_t R = 1; R <= cdiv(sv->max_rows, br); ++R) {
-        const int64_t M = std::min<int64_t>(R * br, sv->max_rows);
-        const int S = plan_for(sv->ctx, sv->idx, M, sv->mode, KP).first;
-        pb = std::max(pb, part_bytes(sv->idx, M, KP, S));
+    constexpr int step = 64;  // the smallest tile: the split rule changes at most this often
+    for (int64_t R = 1; R <= cdiv(sv->max_rows, step); ++R) {
+        const int64_t M = std::min<int64_t>(R * step, sv->max_rows);
+        for (int64_t Mi : {M, std::max<int64_t>(1, M - step + 1)}) {
+            const int S = plan_for(sv->ctx, sv->idx, Mi, sv->mode, KP).S;
+            pb = std::max(pb, part_bytes(sv->idx, Mi, KP, S));
+        }
     }
     if (pbytes_out) *pbytes_out = pb;
     const int64_t M = sv->max_rows;
@@ -161,11 +164,15 @@ int launch_slot(rm_server *sv, Slot *sl) {
         if ((rc = encode_into(sv->ctx, sl->d_words, rows, sv->k, sv->s, sv->m, sv->c, sv->mode,
                               sv->rdim, nullptr, sl->d_q, sl->d_q32, sl->st)))
             return rc;
-        const auto pl = plan_for(sv->ctx, sv->idx, rows, sv->mode, KP);
+        // the context's plan may have changed since the slot was sized (a later
+        // rm_autotune / rm_plan_set): keep at most the splits whose partials
+        // fit the slot -- results do not depend on the split count
+        Plan pl = plan_for(sv->ctx, sv->idx, rows, sv->mode, KP);
+        pl.S = max_splits_within(sv->idx, rows, KP, pl.S, sl->pbytes);
         if ((rc = topk_core_bufs(sv->ctx, sv->idx, sl->d_q, sv->rdim * (elem ? 2 : 1), elem,
                                  sl->d_q32, sv->rdim, rows, sv->ktop, sv->kprime, sv->mode,
-                                 sl->d_vals, sl->d_locs, rows, pl.first, pl.second, sl->d_pv,
-                                 sl->d_pc, sl->pbytes, sl->st)))
+                                 sl->d_vals, sl->d_locs, rows, pl, sl->d_pv, sl->d_pc, sl->pbytes,
+                                 sl->st)))
             return rc;
     }
     RM_HIP(hipMemcpyAsync(sl->h_vals, sl->d_vals, (size_t)rows * sv->ktop * 4,
@@ -181,10 +188,7 @@ void fail_pieces(rm_server *sv, Slot *sl, int code, const std::string &msg) {
     for (const Piece &p : sl->pieces) {
         Req *r = p.r;
         --r->pending;
-        if (r->state == Q_FAILED) {
-            if (r->waited && r->pending == 0) sv->reqs.erase(r->id);
-            continue;
-        }
+        if (r->state == Q_FAILED) continue;
         r->state = Q_FAILED;
         r->err = code;
         r->msg = msg;
@@ -265,8 +269,10 @@ void completer(rm_server *sv) {
     hipSetDevice(sv->ctx->device);
     std::unique_lock<std::mutex> lk(sv->mu);
     for (;;) {
-        sv->cv.wait(lk, [&] { return !sv->inflight.empty() || sv->stop; });
-        if (sv->inflight.empty()) break;  // stop requested and nothing in flight
+        // the dispatcher may still be enqueueing a launch after stop is set:
+        // exit only once it is gone and nothing is in flight
+        sv->cv.wait(lk, [&] { return !sv->inflight.empty() || sv->disp_done; });
+        if (sv->inflight.empty()) break;
         Slot *sl = sv->inflight.front();
         lk.unlock();
         const hipError_t e = hipEventSynchronize(sl->done);
@@ -293,10 +299,7 @@ void completer(rm_server *sv) {
             for (const Piece &p : sl->pieces) {
                 Req *r = p.r;
                 --r->pending;
-                if (r->state == Q_FAILED) {
-                    if (r->waited && r->pending == 0) sv->reqs.erase(r->id);
-                    continue;
-                }
+                if (r->state == Q_FAILED) continue;
                 r->rows_done += p.hi - p.lo;
                 sv->rows_done += p.hi - p.lo;
                 if (r->rows_done == r->n) {
@@ -409,7 +412,9 @@ int rm_server_wait(rm_server *srv, int64_t req_id, int timeout_ms, float *host_v
     auto it = srv->reqs.find(req_id);
     RM_CHECK(it != srv->reqs.end(), RM_ERR_ARG, "unknown request id %lld", (long long)req_id);
     Req *r = it->second.get();
-    auto ready = [&] { return r->state >= Q_DONE; };
+    // a failed request returns once none of its rows is in flight any more:
+    // its words (submit_ref) are no longer read by a queued copy
+    auto ready = [&] { return r->state >= Q_DONE && !(r->state == Q_FAILED && r->pending > 0); };
     if (timeout_ms < 0) {
         srv->cv.wait(lk, ready);
     } else if (!srv->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
@@ -426,10 +431,7 @@ int rm_server_wait(rm_server *srv, int64_t req_id, int timeout_ms, float *host_v
         if (host_vals) std::memcpy(host_vals, r->vals.data(), r->vals.size() * 4);
         if (host_locs) std::memcpy(host_locs, r->locs.data(), r->locs.size() * 4);
     }
-    if (state == Q_FAILED && r->pending > 0)
-        r->waited = true;  // rows still in flight: the completer frees it
-    else
-        srv->reqs.erase(it);
+    srv->reqs.erase(req_id);  // (another waiter of the same id may have erased it)
     if (state == Q_CANCELLED) return fail(RM_ERR_STATE, "request %lld was cancelled", (long long)req_id);
     if (state == Q_FAILED) return fail(err ? err : RM_ERR_HIP, "request %lld failed: %s", (long long)req_id, msg.c_str());
     clear_error();
@@ -491,7 +493,14 @@ int rm_server_destroy(rm_server *srv) {
         srv->queue.clear();
         srv->cv.notify_all();
     }
+    // the dispatcher finishes the launch it may be enqueueing, then exits;
+    // the completer drains every launch in flight before it exits
     if (srv->disp.joinable()) srv->disp.join();
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->disp_done = true;
+        srv->cv.notify_all();
+    }
     if (srv->comp.joinable()) srv->comp.join();
     hipSetDevice(srv->ctx->device);
     for (auto &sl : srv->slots) free_slot(sl.get());

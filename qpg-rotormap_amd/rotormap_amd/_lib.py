@@ -60,9 +60,9 @@ SIGNATURES = {
     "rm_map_batch": (i32, [vp, vp, vp, i64, i32, i32, i32, i32, i32, i32, i32, vp, vp, i64, vp, vp]),
     "rm_map_batches": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, vp, i64,
                              vp]),
-    "rm_plan_splits": (i32, [vp, vp, i64, i32]),
-    "rm_plan_set": (i32, [vp, i64, i32, i32, i32]),
-    "rm_autotune": (i32, [vp, vp, vp, i32, i32, vp, vp, vp]),
+    "rm_plan_get": (i32, [vp, vp, i64, i32, i32, vp, vp, vp]),
+    "rm_plan_set": (i32, [vp, i64, i32, i32, i32, i32]),
+    "rm_autotune": (i32, [vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
     "rm_set_persistent": (i32, [vp, i32]),
     "rm_prof_enable": (i32, [vp, i32]),
     "rm_prof_read": (i32, [vp, vp, vp, vp]),

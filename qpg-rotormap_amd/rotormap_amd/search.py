@@ -126,14 +126,31 @@ class TopKEngine:
         return v, l
 
     def autotune(self, bucket_rows):
-        """Time split counts per batch-size bucket; the context keeps the best."""
+        """Time tile shapes, split counts and XCD groupings per batch-size
+        bucket at this engine's k / kprime; the context keeps the fastest.
+        Returns (tile rows, splits, query groups, ms per launch) per bucket."""
         b = np.ascontiguousarray(bucket_rows, dtype=np.int64)
+        tile = np.zeros(len(b), dtype=np.int32)
         best = np.zeros(len(b), dtype=np.int32)
         qg = np.zeros(len(b), dtype=np.int32)
         ms = np.zeros(len(b), dtype=np.float32)
         L.check(L.lib().rm_autotune(L.ctx(self.device), self.index.handle, L.ptr(b), len(b),
-                                    self.mode, L.ptr(best), L.ptr(qg), L.ptr(ms)))
-        return best, qg, ms
+                                    self.mode, self.k, self.kprime, L.ptr(tile), L.ptr(best),
+                                    L.ptr(qg), L.ptr(ms)))
+        return tile, best, qg, ms
+
+    def plan(self, M):
+        """(tile rows, splits, query groups) a batch of M rows runs with."""
+        t, s, g = C.c_int32(), C.c_int32(), C.c_int32()
+        kp = self.kprime if self.mode == L.RM_MODE_FP8_RESCORE else self.k
+        L.check(L.lib().rm_plan_get(L.ctx(self.device), self.index.handle, int(M), self.mode, kp,
+                                    C.byref(t), C.byref(s), C.byref(g)))
+        return t.value, s.value, g.value
+
+    def set_plan(self, M, tile_rows=0, splits=0, qgroup=0):
+        """Pin (or with all zeros clear) the plan of M's size bucket."""
+        L.check(L.lib().rm_plan_set(L.ctx(self.device), int(M), self.mode, int(tile_rows),
+                                    int(splits), int(qgroup)))
 
 
 def topk_flow(source, eng, re):

@@ -6,7 +6,7 @@ compact key per residue class r of the window start; the bin of class r
 collects sum_{p = r mod P, p < nw} exp(i f theta p), a geometric series in
 exp(i f theta P).  The G-homopolymer key 2*sum_{j<c}4^j is zeroed; normalize
 = 0 divides by the root of the total energy (+1e-16)
-(RotorMap/encode/reference.jl:67-135).
+(RotorMap/encode/reference.jl:37-105).
 Run: python tests/golden/make_golden.py
 """
 import cmath

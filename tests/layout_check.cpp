@@ -8,15 +8,16 @@
 
 int main() {
     int bad = 0;
-    for (int kp = 1; kp <= rm::kMaxKP; ++kp) {
-        const int br = rm::rows_per_tile(kp);
+    for (int kp = 1; kp <= rm::kMaxKP; ++kp)
+      for (int br : {64, 128, 256}) {
+        if (!rm::tile_rows_ok(kp, br)) continue;
         if (rm::uses_ring(kp)) {
             // ring layout: the ring, then candidate slots, tau, counters, 3
             // flags; a merge stages the lists' values and columns in ring
             // buffers 2 and 3 (32 entries per query), the first tile its seed
             // words in buffer 2
-            const size_t total = rm::topk_lds_bytes(kp);
-            const int capd = rm::topk_cand_slots(kp);
+            const size_t total = rm::topk_lds_bytes(kp, br);
+            const int capd = rm::topk_cand_slots(kp, br);
             const size_t cand_end = rm::ring_cand_offset(br) + (size_t)br * (capd + 1) * 8;
             const size_t end = cand_end + rm::tail_bytes(br) + (size_t)br * 4;  // + floor
             if (capd < 1 || rm::ring_cand_offset(br) < rm::ring_bytes(br) || end > total ||
@@ -30,7 +31,8 @@ int main() {
             }
             continue;
         }
-        const size_t total = rm::topk_lds_bytes(kp);
+        if (rm::ring_list(kp) && br != 256) continue;  // unswizzled layouts: one tile of 256
+        const size_t total = rm::topk_lds_bytes(kp, br);
         const int capd = rm::cand_slots(br, kp);
         // regions in kernel order: stage buffers, lists, candidate slots, tau,
         // counters, 3 flags

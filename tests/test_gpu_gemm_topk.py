@@ -93,17 +93,23 @@ def test_gemm_chunk_fp16_vs_fp64(torch_gpu):
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-6
 
 
-def fused(torch, eng, Q, Q32=None, splits=None, qg=0):
+TILES = (256, 128, 64)  # query rows per tile of the ring kernels (k' <= 32)
+
+
+def fused(torch, eng, Q, Q32=None, splits=None, qg=0, tile=0):
+    """One fused launch under a pinned plan (tile 0 = the default tile)."""
     M = Q.shape[0]
-    mode = eng.mode
-    if splits is not None:
-        L.check(R.lib().rm_plan_set(L.ctx(0), M, mode, splits, qg))
+    pinned = splits is not None or tile
+    if pinned:
+        eng.set_plan(M, tile, splits or 0, qg)
+        if tile:
+            assert eng.plan(M)[0] == tile
     dq = torch.from_numpy(Q).cuda()
     d32 = torch.from_numpy(Q32).cuda() if Q32 is not None else None
     v, l = eng.batch_gemm_topk(dq, d32)
     torch.cuda.synchronize()
-    if splits is not None:
-        L.check(R.lib().rm_plan_set(L.ctx(0), M, mode, 0, 0))
+    if pinned:
+        eng.set_plan(M, 0, 0, 0)
     return v.cpu().numpy(), l.cpu().numpy()
 
 
@@ -118,13 +124,15 @@ def test_fused_topk_bitexact(torch_gpu, M, N):
     ev, el = O.topk_rows(C, k)
     eng = R.TopKEngine(B, k=k, mode="fp8")
     tiles = -(-N // 256)
-    for S in sorted({1, 2, 3, tiles}):
-        if S > tiles:
-            continue
-        for qg in (0, 2, 3):  # query tiles grouped per XCD (0 = all)
-            v, l = fused(torch, eng, Q, splits=S, qg=qg)
-            np.testing.assert_array_equal(l, el, err_msg=f"locs, splits={S}, qg={qg}")
-            np.testing.assert_array_equal(v, ev, err_msg=f"vals, splits={S}, qg={qg}")
+    for tile in TILES:
+        for S in sorted({1, 2, 3, tiles}):
+            if S > tiles:
+                continue
+            for qg in (0, 2, 3):  # query tiles grouped per XCD (0 = all)
+                v, l = fused(torch, eng, Q, splits=S, qg=qg, tile=tile)
+                msg = f"tile={tile}, splits={S}, qg={qg}"
+                np.testing.assert_array_equal(l, el, err_msg="locs, " + msg)
+                np.testing.assert_array_equal(v, ev, err_msg="vals, " + msg)
 
 
 def test_fused_topk_ties_and_k64(torch_gpu):
@@ -135,15 +143,16 @@ def test_fused_topk_ties_and_k64(torch_gpu):
     B = np.concatenate([base] * 30)  # 1200 columns, each value 30 times
     Q = rand_e4m3((M, K), 0.05, 12)
     C = gemm_c(torch, Q, B)
-    # k <= 32: 256-query tiles; 33..48: 128-query tiles, rank merge; 64:
-    # 128-query tiles, sequential merge
+    # k <= 32: the ring tiles of 256, 128 and 64 queries; 33..64: the
+    # 128-query register-staged tile
     for k in (20, 32, 40, 48, 64):
         ev, el = O.topk_rows(C, k)
         eng = R.TopKEngine(B, k=k, mode="fp8")
-        for S in (1, 4):
-            v, l = fused(torch, eng, Q, splits=S)
-            np.testing.assert_array_equal(l, el)
-            np.testing.assert_array_equal(v, ev)
+        for tile in (TILES if k <= 32 else (0,)):
+            for S in (1, 4):
+                v, l = fused(torch, eng, Q, splits=S, tile=tile)
+                np.testing.assert_array_equal(l, el, err_msg=f"k={k} tile={tile} S={S}")
+                np.testing.assert_array_equal(v, ev, err_msg=f"k={k} tile={tile} S={S}")
 
 
 def test_fused_topk_fp16_bitexact(torch_gpu):
@@ -158,10 +167,11 @@ def test_fused_topk_fp16_bitexact(torch_gpu):
     torch.cuda.synchronize()
     ev, el = O.topk_rows(C.cpu().numpy(), 20)
     eng = R.TopKEngine(B8, k=20, mode="fp16")
-    for S in (1, 3):
-        v, l = fused(torch, eng, Q, splits=S)
-        np.testing.assert_array_equal(l, el)
-        np.testing.assert_array_equal(v, ev)
+    for tile in TILES:
+        for S in (1, 3):
+            v, l = fused(torch, eng, Q, splits=S, tile=tile)
+            np.testing.assert_array_equal(l, el, err_msg=f"tile={tile} S={S}")
+            np.testing.assert_array_equal(v, ev, err_msg=f"tile={tile} S={S}")
 
 
 @pytest.mark.parametrize("K", [64, 128, 192, 256, 320])
@@ -176,10 +186,11 @@ def test_fused_topk_short_k_bitexact(torch_gpu, K):
     B = rand_e4m3((N, K), 0.05, 60 + K)
     ev, el = O.topk_rows(gemm_c(torch, Q, B), k)
     eng = R.TopKEngine(B, k=k, mode="fp8")
-    for S in (1, 3):
-        v, l = fused(torch, eng, Q, splits=S)
-        np.testing.assert_array_equal(l, el, err_msg=f"fp8 K={K} splits={S}")
-        np.testing.assert_array_equal(v, ev, err_msg=f"fp8 K={K} splits={S}")
+    for tile in TILES:
+        for S in (1, 3):
+            v, l = fused(torch, eng, Q, splits=S, tile=tile)
+            np.testing.assert_array_equal(l, el, err_msg=f"fp8 K={K} tile={tile} splits={S}")
+            np.testing.assert_array_equal(v, ev, err_msg=f"fp8 K={K} tile={tile} splits={S}")
     Q16 = (np.random.default_rng(70 + K).standard_normal((M, K)) * 0.05).astype(np.float16)
     B16 = O.e4m3_table()[B].astype(np.float16)
     C = torch.zeros(N, M, dtype=torch.float32, device="cuda").t()
@@ -187,10 +198,11 @@ def test_fused_topk_short_k_bitexact(torch_gpu, K):
     torch.cuda.synchronize()
     ev, el = O.topk_rows(C.cpu().numpy(), k)
     eng16 = R.TopKEngine(B, k=k, mode="fp16")
-    for S in (1, 3):
-        v, l = fused(torch, eng16, Q16, splits=S)
-        np.testing.assert_array_equal(l, el, err_msg=f"fp16 K={K} splits={S}")
-        np.testing.assert_array_equal(v, ev, err_msg=f"fp16 K={K} splits={S}")
+    for tile in TILES:
+        for S in (1, 3):
+            v, l = fused(torch, eng16, Q16, splits=S, tile=tile)
+            np.testing.assert_array_equal(l, el, err_msg=f"fp16 K={K} tile={tile} splits={S}")
+            np.testing.assert_array_equal(v, ev, err_msg=f"fp16 K={K} tile={tile} splits={S}")
 
 
 def test_fused_topk_random_plans_bitexact(torch_gpu):
@@ -208,6 +220,7 @@ def test_fused_topk_random_plans_bitexact(torch_gpu):
         tiles = -(-N // 256)
         S = int(rng.integers(1, min(tiles, 9) + 1))
         qg = int(rng.choice([0, 1, 2, 3]))
+        tile = int(rng.choice(TILES))
         B = rand_e4m3((N, K), 0.05, 100 + case)
         if fp16:
             Q = (rng.standard_normal((M, K)) * 0.05).astype(np.float16)
@@ -221,8 +234,8 @@ def test_fused_topk_random_plans_bitexact(torch_gpu):
             C = gemm_c(torch, Q, B)
         ev, el = O.topk_rows(C, k)
         eng = R.TopKEngine(B, k=k, mode="fp16" if fp16 else "fp8")
-        v, l = fused(torch, eng, Q, splits=S, qg=qg)
-        msg = f"case {case}: M={M} N={N} K={K} k={k} S={S} qg={qg} fp16={fp16}"
+        v, l = fused(torch, eng, Q, splits=S, qg=qg, tile=tile)
+        msg = f"case {case}: M={M} N={N} K={K} k={k} S={S} qg={qg} tile={tile} fp16={fp16}"
         np.testing.assert_array_equal(l, el, err_msg=msg)
         np.testing.assert_array_equal(v, ev, err_msg=msg)
 
@@ -235,11 +248,12 @@ def test_fused_rescore_bitexact(torch_gpu):
     C = gemm_c(torch, Q8, B)
     _, cand = O.topk_rows(C, kp)
     eng = R.TopKEngine(B, k=k, kprime=kp, mode="fp8_rescore")
-    v, l = fused(torch, eng, Q8, Q32)
-    for r in range(M):
-        ev, el = O.rescore_rank(Q32[r], B, cand[r], k)
-        np.testing.assert_array_equal(l[r], el)
-        np.testing.assert_array_equal(v[r], ev)
+    for tile in TILES:
+        v, l = fused(torch, eng, Q8, Q32, tile=tile)
+        for r in range(M):
+            ev, el = O.rescore_rank(Q32[r], B, cand[r], k)
+            np.testing.assert_array_equal(l[r], el, err_msg=f"tile={tile} row {r}")
+            np.testing.assert_array_equal(v[r], ev, err_msg=f"tile={tile} row {r}")
     # rescored values vs the FP64 dot of the fp32 query with the stored columns
     deq = O.e4m3_table()[B].astype(np.float64)
     ref = Q32.astype(np.float64) @ deq.T
@@ -283,10 +297,11 @@ def test_persistent_grid_bitexact(torch_gpu):
     try:
         for on in (1, 0):
             L.check(R.lib().rm_set_persistent(ctx, on))
-            for S in (1, 16, 128):
-                v, l = fused(torch, eng, Q, splits=S, qg=0)
-                np.testing.assert_array_equal(l, el, err_msg=f"locs, persistent={on}, S={S}")
-                np.testing.assert_array_equal(v, ev, err_msg=f"vals, persistent={on}, S={S}")
+            for tile, S in ((256, 1), (256, 16), (256, 128), (64, 8), (128, 32)):
+                v, l = fused(torch, eng, Q, splits=S, qg=0, tile=tile)
+                msg = f"persistent={on}, tile={tile}, S={S}"
+                np.testing.assert_array_equal(l, el, err_msg="locs, " + msg)
+                np.testing.assert_array_equal(v, ev, err_msg="vals, " + msg)
     finally:
         L.check(R.lib().rm_set_persistent(ctx, 1))
 
@@ -319,11 +334,13 @@ def test_index_past_2_32_elements(torch_gpu):
 
 
 def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
-    """With no tuned plan, the split count is the smallest (from 'one item
-    per CU' up, doubling) for which the query tiles worked on at once,
-    min(R, CUs / S) tiles of 256 rows, fit in 128 MB: a 65,536-read e4m3 batch
-    (2 KB rows, 128 MB) stays unsplit on 256 CUs; the same batch in fp16 (4 KB
-    rows, 256 MB) takes two splits; a 4,096-read batch takes CUs / 16."""
+    """With no tuned plan, a batch takes the smallest ring tile that holds it
+    (64 rows up to 64, 128 up to 128, else 256) and the smallest split count
+    (from 'one item per CU' up, doubling) for which the query tiles worked on
+    at once, min(R, CUs / S) tiles of that many rows, fit in 128 MB: a
+    65,536-read e4m3 batch (2 KB rows, 128 MB) stays unsplit on 256 CUs; the
+    same batch in fp16 (4 KB rows, 256 MB) takes two splits; a 4,096-read
+    batch takes CUs / 16."""
     torch = torch_gpu
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     idx = R.ResidentIndex(np.zeros((512 * 256, 2048), np.uint8), fp16=True)
@@ -331,21 +348,35 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
     lib = R.lib()
     try:
         for mode in (L.RM_MODE_FP8, L.RM_MODE_FP8_RESCORE, L.RM_MODE_FP16):
-            for M in (65_536, 4_096):
-                L.check(lib.rm_plan_set(ctx, M, mode, 0, 0))
+            for M in (65_536, 4_096, 100, 30):
+                L.check(lib.rm_plan_set(ctx, M, mode, 0, 0, 0))
 
-        def expect(M, row_bytes):
-            rt = -(-M // 256)
+        def expect(M, row_bytes, kp=20):
+            bq = 128 if kp > 32 else (64 if M <= 64 else 128 if M <= 128 else 256)
+            rt = -(-M // bq)
             S = max(1, min(cus // rt, 256))
-            while S < 256 and min(rt, -(-cus // S)) * 256 * row_bytes > 128 << 20:
+            while S < 256 and min(rt, -(-cus // S)) * bq * row_bytes > 128 << 20:
                 S = min(2 * S, 256)
-            return S
+            return bq, S, 0
 
-        s8 = lib.rm_plan_splits(ctx, idx.handle, 65_536, L.RM_MODE_FP8)
-        s16 = lib.rm_plan_splits(ctx, idx.handle, 65_536, L.RM_MODE_FP16)
-        assert (s8, s16) == (expect(65_536, 2048), expect(65_536, 4096))
+        def plan(M, mode, kp=20):
+            t, s_, g = L.C.c_int32(), L.C.c_int32(), L.C.c_int32()
+            L.check(lib.rm_plan_get(ctx, idx.handle, M, mode, kp, L.C.byref(t), L.C.byref(s_),
+                                    L.C.byref(g)))
+            return t.value, s_.value, g.value
+
+        p8, p16 = plan(65_536, L.RM_MODE_FP8), plan(65_536, L.RM_MODE_FP16)
+        assert (p8, p16) == (expect(65_536, 2048), expect(65_536, 4096))
         if cus == 256:
-            assert (s8, s16) == (1, 2)
-        assert lib.rm_plan_splits(ctx, idx.handle, 4_096, L.RM_MODE_FP8) == expect(4_096, 2048)
+            assert (p8[1], p16[1]) == (1, 2)
+        for M in (4_096, 100, 30):
+            assert plan(M, L.RM_MODE_FP8) == expect(M, 2048), M
+        assert plan(30, L.RM_MODE_FP8, kp=48) == expect(30, 2048, kp=48)
+        # a pinned plan; a tile the list length cannot run falls back
+        L.check(lib.rm_plan_set(ctx, 4_096, L.RM_MODE_FP8, 64, 8, 2))
+        assert plan(4_096, L.RM_MODE_FP8) == (64, 8, 2)
+        assert plan(4_096, L.RM_MODE_FP8, kp=48)[0] == 128
+        L.check(lib.rm_plan_set(ctx, 4_096, L.RM_MODE_FP8, 0, 0, 0))
+        assert plan(4_096, L.RM_MODE_FP8) == expect(4_096, 2048)
     finally:
         idx.close()
