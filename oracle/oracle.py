@@ -48,6 +48,10 @@ def lib():
             "rmo_rescore_dot": (C.c_float, [vp, vp, i32]),
             "rmo_rescore_rank": (None, [vp, vp, i32, vp, i32, i32, vp, vp]),
             "rmo_fasta_reads": (i64, [vp, i64, i32, vp, i64, vp, vp]),
+            "rmo_cpu_map": (None, [vp, i64, i32, i32, i32, i32, vp, i64, i32, i32, vp, vp, i32]),
+            "rmo_golden_query": (None, [vp, i32, i32, i32, i32, i32, vp]),
+            "rmo_dots_f64": (None, [vp, vp, i32, i64, vp, i32, vp]),
+            "rmo_scan_topk_f64": (None, [vp, i64, vp, i32, i64, i64, i32, vp, vp, i32]),
         }
         for n, (r, a) in sig.items():
             f = getattr(o, n)
@@ -209,3 +213,53 @@ def fasta_reads(raw, k):
                               _p(hoff), _p(hlen))
     heads = [raw[o:o + l].tobytes() for o, l in zip(hoff[:n], hlen[:n])]
     return words[:n], heads
+
+
+CHAINS = {"e4m3": 0, "f32": 1, "f16": 2, "f64": 3}
+
+
+def golden_query(words, k, s, m, c, chain="e4m3"):
+    """The golden FP64 encoder's output of one read through a query storage
+    chain ('e4m3' | 'f32' | 'f16' | 'f64'), as float64."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    q = np.empty(2 * m * 4 ** c)
+    lib().rmo_golden_query(_p(w), k, s, m, c, CHAINS[chain], _p(q))
+    return q
+
+
+def dots_f64(q, B, cols):
+    """FP64 dots of q with the 1-based columns `cols` of B (uint8 e4m3 or
+    float16 rows); column 0 gives NaN."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    elem = 1 if B.dtype == np.float16 else 0
+    Bv = B.view(np.uint16) if elem else B
+    c = np.ascontiguousarray(cols, dtype=np.int32)
+    out = np.empty(len(c))
+    lib().rmo_dots_f64(_p(q), _p(Bv), elem, len(q), _p(c), len(c), _p(out))
+    return out
+
+
+def scan_topk_f64(Q, B, ktop, threads=None):
+    """Full-scan FP64 top-k of queries Q (n, K) over all rows of B (uint8
+    e4m3 or float16): (FP64 values, 1-based locs)."""
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    n, K = Q.shape
+    elem = 1 if B.dtype == np.float16 else 0
+    Bv = np.ascontiguousarray(B).view(np.uint16) if elem else np.ascontiguousarray(B)
+    v = np.empty((n, ktop))
+    l = np.empty((n, ktop), dtype=np.int32)
+    lib().rmo_scan_topk_f64(_p(Q), n, _p(Bv), elem, Bv.shape[0], K, ktop, _p(v), _p(l),
+                            threads or os.cpu_count() or 1)
+    return v, l
+
+
+def cpu_map(words, k, s, m, c, B, ktop, threads=None):
+    """The CPU golden mapping path: (vals float32 (n, ktop), locs 1-based)."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    n = w.shape[0]
+    B = np.ascontiguousarray(B, dtype=np.uint8)
+    v = np.empty((n, ktop), dtype=np.float32)
+    l = np.empty((n, ktop), dtype=np.int32)
+    lib().rmo_cpu_map(_p(w), n, k, s, m, c, _p(B), B.shape[0], B.shape[1], ktop, _p(v), _p(l),
+                      threads or os.cpu_count() or 1)
+    return v, l

@@ -27,6 +27,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define RMO_S30 1073741824.0 /* 2^30: fixed-point scale of the phasor table */
 
@@ -532,4 +535,125 @@ void rmo_cpu_map(const uint32_t *words, int64_t n, int k, int s, int m, int c,
         }
         free(codes); free(emb); free(q);
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Score models of the three GPU modes (tests and the bench's score error)   */
+/* ------------------------------------------------------------------------ */
+
+/* The golden FP64 encoder's output of one read (words: ceil(k/16) uint32)
+ * rounded through a query storage chain:
+ *   0  e4m3(fp32(fp16(fp32(x))))  the FP8 query (engine_fp8.jl:365-368,
+ *                                 the hrefq of engine_fp8.jl:873)
+ *   1  fp32(x)                    the rescoring query
+ *   2  fp16(fp32(x))              the FP16 query (engine_fp16.jl upload)
+ *   3  x                          the unquantised FP64 embedding
+ * normalize = 0, the mapping flow's mode (RotorMap/index/load.jl:57-59). */
+void rmo_golden_query(const uint32_t *words, int k, int s, int m, int c, int chain, double *q) {
+    const int K = 2 * m * (1 << (2 * c));
+    uint8_t *codes = (uint8_t *)malloc((size_t)k);
+    double nr[64];
+    rmo_frag_codes(words, k, codes);
+    rmo_ref_rope_frag_real(codes, k, s, m, c, 0, q, nr);
+    for (int j = 0; j < K; ++j) {
+        const float f = (float)q[j];
+        if (chain == 0) q[j] = rmo_e4m3_to_f32(rmo_f64_to_e4m3_chain(q[j]));
+        else if (chain == 1) q[j] = f;
+        else if (chain == 2) q[j] = rmo_f16_to_f32(rmo_f32_to_f16(f));
+    }
+    free(codes);
+}
+
+/* out[j] = sum_k q[k] * B[cols[j]-1][k] in FP64 over the stored column
+ * (elem 0: e4m3 bytes, 1: binary16 bits), sequential in k; column id 0 (an
+ * empty slot) gives NAN. */
+void rmo_dots_f64(const double *q, const void *B, int elem, int64_t K, const int32_t *cols, int n,
+                  double *out) {
+    for (int j = 0; j < n; ++j) {
+        if (cols[j] <= 0) { out[j] = NAN; continue; }
+        const int64_t o = (int64_t)(cols[j] - 1) * K;
+        double acc = 0.0;
+        if (elem == 0) {
+            const uint8_t *b = (const uint8_t *)B + o;
+            for (int64_t t = 0; t < K; ++t) acc += q[t] * (double)rmo_e4m3_to_f32(b[t]);
+        } else {
+            const uint16_t *b = (const uint16_t *)B + o;
+            for (int64_t t = 0; t < K; ++t) acc += q[t] * (double)rmo_f16_to_f32(b[t]);
+        }
+        out[j] = acc;
+    }
+}
+
+/* Insert (x, col) into a sorted list (v, vd, l) of ktop entries holding fill. */
+static void rmo_list_insert(float *v, double *vd, int32_t *l, int ktop, int *fill, float x,
+                            double xd, int32_t cj) {
+    if (*fill == ktop && !rmo_beats(x, cj, v[ktop - 1], l[ktop - 1])) return;
+    int pos = (*fill < ktop) ? *fill : ktop - 1;
+    while (pos > 0 && rmo_beats(x, cj, v[pos - 1], l[pos - 1])) {
+        v[pos] = v[pos - 1]; vd[pos] = vd[pos - 1]; l[pos] = l[pos - 1]; --pos;
+    }
+    v[pos] = x; vd[pos] = xd; l[pos] = cj;
+    if (*fill < ktop) ++*fill;
+}
+
+/* Full-scan FP64 top-ktop of n queries (row-major n x K doubles) over all N
+ * stored columns (elem as rmo_dots_f64), ranked by (float(score) desc,
+ * column asc) like rmo_cpu_map; vals are the FP64 scores.  OpenMP over
+ * column blocks (each column is read once for all queries); the per-block
+ * lists merge into the same result in any order, since (score, column)
+ * keys are distinct. */
+void rmo_scan_topk_f64(const double *Q, int64_t n, const void *B, int elem, int64_t N, int64_t K,
+                       int ktop, double *vals, int32_t *locs, int nthreads) {
+    float lut[256];
+    for (int i = 0; i < 256; ++i) lut[i] = rmo_e4m3_to_f32((uint8_t)i);
+    if (nthreads < 1) nthreads = 1;
+    const int64_t per = (int64_t)n * ktop;
+    float *tv = (float *)malloc(sizeof(float) * (size_t)(per * nthreads));
+    double *tvd = (double *)malloc(sizeof(double) * (size_t)(per * nthreads));
+    int32_t *tl = (int32_t *)malloc(sizeof(int32_t) * (size_t)(per * nthreads));
+    int *tf = (int *)calloc((size_t)(n * nthreads), sizeof(int));
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+        const int t = 0, nt = 1;
+#endif
+        float *v = tv + per * t;
+        double *vd = tvd + per * t;
+        int32_t *l = tl + per * t;
+        int *fill = tf + n * t;
+        double *bcol = (double *)malloc(sizeof(double) * (size_t)K);
+        for (int64_t col = t; col < N; col += nt) {
+            if (elem == 0) {
+                const uint8_t *b = (const uint8_t *)B + col * K;
+                for (int64_t x = 0; x < K; ++x) bcol[x] = lut[b[x]];
+            } else {
+                const uint16_t *b = (const uint16_t *)B + col * K;
+                for (int64_t x = 0; x < K; ++x) bcol[x] = rmo_f16_to_f32(b[x]);
+            }
+            for (int64_t r = 0; r < n; ++r) {
+                const double *q = Q + r * K;
+                double acc = 0.0;
+                for (int64_t x = 0; x < K; ++x) acc += q[x] * bcol[x];
+                rmo_list_insert(v + r * ktop, vd + r * ktop, l + r * ktop, ktop, fill + r,
+                                (float)acc, acc, (int32_t)(col + 1));
+            }
+        }
+        free(bcol);
+    }
+    for (int64_t r = 0; r < n; ++r) {
+        float *v = (float *)malloc(sizeof(float) * (size_t)ktop);
+        double *vd = vals + r * ktop;
+        int32_t *l = locs + r * ktop;
+        int fill = 0;
+        for (int j = 0; j < ktop; ++j) { v[j] = -INFINITY; vd[j] = -INFINITY; l[j] = 0; }
+        for (int t = 0; t < nthreads; ++t)
+            for (int j = 0; j < tf[n * t + r]; ++j) {
+                const int64_t e = per * t + r * ktop + j;
+                rmo_list_insert(v, vd, l, ktop, &fill, tv[e], tvd[e], tl[e]);
+            }
+        free(v);
+    }
+    free(tv); free(tvd); free(tl); free(tf);
 }

@@ -145,3 +145,24 @@ def test_server_trace(torch_gpu, setup):
     for r, o, m in zip(res["results"], res["offsets"], np.minimum(n, 3000)):
         np.testing.assert_array_equal(r.locs, l0[o:o + m])
     print(f"[serve] trace p50 {res['p50_ms']:.2f} ms p99 {res['p99_ms']:.2f} ms")
+
+
+def test_server_after_plan_change(torch_gpu, setup):
+    """A server sizes its slots' split partials from the plans of its
+    creation; a plan set later (rm_plan_set / rm_autotune) with more splits
+    than a slot holds runs with the split count that fits, and the rows
+    still equal one-shot (the split count never changes a result)."""
+    ref, re, idx, reads = setup
+    B = R.ResidentIndex(idx.embeds8)
+    eng = R.TopKEngine(B, k=20, kprime=32, mode="fp8_rescore")
+    v0, l0 = oneshot(torch_gpu, idx, re, reads.words[:1500], "fp8_rescore")
+    try:
+        with R.Server(B, re, max_rows=700, n_streams=2) as srv:
+            for M in (700, 100):  # the buckets of a full slot and of the tail launch
+                eng.set_plan(M, 64, 256, 0)
+            out = srv.wait(*srv.submit(reads.words[:1500]))
+    finally:
+        for M in (700, 100):
+            eng.set_plan(M, 0, 0, 0)
+    np.testing.assert_array_equal(out.vals, v0)
+    np.testing.assert_array_equal(out.locs, l0)

@@ -311,3 +311,76 @@ def test_rescore_dot_fixed_order():
     got = O.rescore_dot(q, b)
     assert got == s[0]
     assert abs(got - float(np.dot(q.astype(np.float64), d.astype(np.float64)))) < 1e-4
+
+
+# ---- the CPU golden mapping path and the modes' score models --------------
+
+def _tiny_map_case(seed=5, k=300, s=4, m=1, c=4, n=6, N=90):
+    """Random reads, a random e4m3 database plus the reads' own encodings as
+    columns (so each read has a clear best hit)."""
+    rng = np.random.default_rng(seed)
+    W = (k + 15) // 16
+    codes = rng.integers(0, 4, (n, k)).astype(np.uint8)
+    words = np.stack([O.forward_pack(cd) for cd in codes])
+    K = 2 * m * 4 ** c
+    hits = [O.store_chain(O.ref_rope_frag_real(cd, k, s, m, c, 0)[0], "e4m3") for cd in codes]
+    B = O.f32_to_e4m3((rng.standard_normal(N * K) * 0.04).astype(np.float32)).reshape(N, K)
+    B[rng.choice(N, n, replace=False)] = np.stack(hits)
+    return words, codes, B, (k, s, m, c)
+
+
+def _numpy_topk(scores, ktop):
+    order = np.lexsort((np.arange(len(scores)), -scores.astype(np.float32)))[:ktop]
+    return scores[order], order + 1
+
+
+def test_cpu_map_vs_numpy_bruteforce():
+    """rmo_cpu_map (the CPU baseline) = golden FP64 encoder -> e4m3 chain ->
+    FP64 dots over the dequantised columns -> total-order top-k, against the
+    same model written with numpy: the golden encoder of the oracle's
+    python-checked restatement, numpy's e4m3 table and float64 matmul."""
+    words, codes, B, (k, s, m, c) = _tiny_map_case()
+    vals, locs = O.cpu_map(words, k, s, m, c, B, 10, threads=2)
+    deq = O.e4m3_table()[B].astype(np.float64)
+    for r in range(len(codes)):
+        h, _ = O.ref_rope_frag_real(codes[r], k, s, m, c, 0)
+        q = O.e4m3_table()[O.store_chain(h, "e4m3")].astype(np.float64)
+        ev, el = _numpy_topk(deq @ q, 10)
+        np.testing.assert_array_equal(locs[r], el)
+        np.testing.assert_allclose(vals[r], ev.astype(np.float32), rtol=0, atol=1e-6)
+        # the read's own column is its best hit (self product = unit energy
+        # up to the e4m3 rounding)
+        assert abs(vals[r, 0] - 1.0) < 0.05
+
+
+def test_golden_query_chains_and_dots():
+    words, codes, B, (k, s, m, c) = _tiny_map_case(seed=8)
+    h, _ = O.ref_rope_frag_real(codes[0], k, s, m, c, 0)
+    np.testing.assert_array_equal(O.golden_query(words[0], k, s, m, c, "f64"), h)
+    np.testing.assert_array_equal(O.golden_query(words[0], k, s, m, c, "f32"),
+                                  h.astype(np.float32).astype(np.float64))
+    np.testing.assert_array_equal(O.golden_query(words[0], k, s, m, c, "f16"),
+                                  h.astype(np.float32).astype(np.float16).astype(np.float64))
+    q8 = O.golden_query(words[0], k, s, m, c, "e4m3")
+    np.testing.assert_array_equal(q8, O.e4m3_table()[O.store_chain(h, "e4m3")].astype(np.float64))
+    cols = np.array([1, 0, 5, len(B)], np.int32)
+    got = O.dots_f64(q8, B, cols)
+    deq = O.e4m3_table()[B].astype(np.float64)
+    np.testing.assert_allclose(got[[0, 2, 3]], deq[[0, 4, len(B) - 1]] @ q8, rtol=0, atol=1e-12)
+    assert np.isnan(got[1])
+    B16 = deq.astype(np.float16)
+    np.testing.assert_allclose(O.dots_f64(q8, B16, cols[[0, 2]]),
+                               B16.astype(np.float64)[[0, 4]] @ q8, rtol=0, atol=1e-12)
+
+
+def test_scan_topk_f64_vs_numpy():
+    words, codes, B, (k, s, m, c) = _tiny_map_case(seed=9)
+    Q = np.stack([O.golden_query(w, k, s, m, c, "f32") for w in words])
+    deq = O.e4m3_table()[B].astype(np.float64)
+    for Bx in (B, deq.astype(np.float16)):
+        v, l = O.scan_topk_f64(Q, Bx, 7, threads=2)
+        full = Q @ deq.T  # the e4m3 -> fp16 widening is exact
+        for r in range(len(Q)):
+            ev, el = _numpy_topk(full[r], 7)
+            np.testing.assert_array_equal(l[r], el)
+            np.testing.assert_allclose(v[r], ev, rtol=0, atol=1e-12)
