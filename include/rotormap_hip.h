@@ -160,6 +160,24 @@ int rm_map_batch(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_
                  int s, int m, int c, int ktop, int kprime, int mode, float *vals,
                  int32_t *locs, int64_t ldo, float *norms, void *stream);
 
+/* rm_map_batch without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as rm_map_batch (bit for bit); its work buffers --
+ * the phasor table, the encoded queries and the split partials -- live in a
+ * static device array of RM_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends, and it calls no hipMalloc / hipFree.  A batch whose buffers
+ * do not fit returns RM_ERR_SHAPE (at rdim 2048 in RM_MODE_FP8_RESCORE about
+ * 6,300 rows), as does a phasor table past 2^18 entries (2*m*(k-s+1)); the
+ * split count is lowered to what fits (results do not depend on it).  Every
+ * call on one GPU shares that GPU's array: calls that overlap on a GPU (two
+ * streams or threads) race.  The first call with a new (k, s, m) stages its
+ * table and synchronizes `stream`. */
+#define RM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int rm_map_batch_noalloc(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_t n,
+                         int k, int s, int m, int c, int ktop, int kprime, int mode, float *vals,
+                         int32_t *locs, int64_t ldo, float *norms, void *stream);
+
 /* Uneven batches in ONE launch: batch b holds rows [offs[b], offs[b+1]) of
  * the concatenated words (host offs, nb+1 entries); results land at the same
  * rows of vals/locs.  Rows are independent, so the batches are encoded

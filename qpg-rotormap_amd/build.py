@@ -28,6 +28,7 @@ SOURCES = [
     ("encode.hip", "hip"),
     ("gemm_topk.hip", "hip"),
     ("fasta.hip", "hip"),
+    ("noalloc.hip", "hip"),
     ("capi.cpp", "hipcc-host"),
     ("serve.cpp", "hipcc-host"),
     ("synth.cpp", "host"),

@@ -245,6 +245,8 @@ int rm_ctx_create(int device, rm_ctx **out) {
     RM_CHECK(device >= 0 && device < ndev, RM_ERR_ARG, "device %d out of range (%d devices)",
              device, ndev);
     RM_HIP(hipSetDevice(device));
+    int rc;
+    if ((rc = noalloc_setup(device))) return rc;  // the scratch array's address on this device
     rm_ctx *c = new rm_ctx();
     c->device = device;
     hipDeviceProp_t prop;

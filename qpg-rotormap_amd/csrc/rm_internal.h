@@ -195,6 +195,10 @@ int launch_widen_f16(uint16_t *dst, const uint8_t *src, int64_t n, hipStream_t s
 
 void build_phase_table(int k, int s, int m, int32_t *tab);
 
+// once per device with it current: the device address of the no-allocation
+// entry point's scratch array (noalloc.hip)
+int noalloc_setup(int device);
+
 // FASTA bytes -> packed reads (fasta.hip)
 int fasta_reads(rm_ctx *ctx, const uint8_t *raw, int64_t n, int k, uint32_t *words,
                 int64_t max_reads, int64_t *hdr_off, int32_t *hdr_len, int64_t *n_reads,

@@ -58,6 +58,8 @@ SIGNATURES = {
     "rm_index_f16": (vp, [vp]),
     "rm_topk_batch": (i32, [vp, vp, vp, i64, vp, i64, i64, i32, i32, i32, vp, vp, i64, vp]),
     "rm_map_batch": (i32, [vp, vp, vp, i64, i32, i32, i32, i32, i32, i32, i32, vp, vp, i64, vp, vp]),
+    "rm_map_batch_noalloc": (i32, [vp, vp, vp, i64, i32, i32, i32, i32, i32, i32, i32, vp, vp, i64,
+                                   vp, vp]),
     "rm_map_batches": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, vp, i64,
                              vp]),
     "rm_plan_get": (i32, [vp, vp, i64, i32, i32, vp, vp, vp]),
