@@ -11,7 +11,7 @@
  *                           (cis(theta*p) in FP64, no recurrence);
  *   rmo_rope_fixed          the deterministic fixed-point encoder the HIP
  *                           kernel implements (integer phasor table, int64
- *                           bin sums, FP64 normalisation in a fixed order).
+ *                           bin sums, FP32 normalisation in a fixed order).
  *                           The HIP kernel must match it bit for bit; it is
  *                           itself pinned to rmo_ref_rope_frag_real by
  *                           tolerance and to closed forms by the tests.
@@ -231,12 +231,13 @@ void rmo_phase_table(int k, int s, int m, int32_t *tab) {
     }
 }
 
-/* Sum of e[idx(j)] in the kernel's fixed order: 64 lane partials (lane l adds
- * items l, l+64, ... sequentially) folded by a halving tree (off = 32..1). */
-static double rmo_tree_sum(const double *e, int D, int m, int idm) {
-    double part[64];
+/* Sum of e[idx(j)] in the kernel's fixed order, in fp32: 64 lane partials
+ * (lane l adds items l, l+64, ... sequentially) folded by a halving tree
+ * (off = 32..1). */
+static float rmo_tree_sum_f32(const float *e, int D, int m, int idm) {
+    float part[64];
     for (int l = 0; l < 64; ++l) {
-        double acc = 0.0;
+        float acc = 0.0f;
         for (int j = l; idm + (int64_t)j * m < D; j += 64) acc = acc + e[idm + j * m];
         part[l] = acc;
     }
@@ -245,9 +246,15 @@ static double rmo_tree_sum(const double *e, int D, int m, int idm) {
     return part[0];
 }
 
-/* out_f64[2D] holds the normalised values before storage rounding; the
- * storage conversions (fp32, fp16, e4m3) are applied by the caller.
- * acc_out (optional, 2D int64, interleaved re/im) returns the bin sums. */
+/* The encoder the HIP kernel implements, bit for bit: integer bin sums of
+ * the phasor table (exact, any order), then FP32 normalisation with one
+ * correctly rounded operation at a time, the reference kernel's precision
+ * (RotorMap/encode/kernel.jl:189-279) with its epsilons (1e-16 for mode 0,
+ * 1e-7 for mode 1, eps() = 2^-52 for modes 2-3): bin value = fp32(sum) *
+ * 2^-30; energies per copy in the fixed tree order; norms and division in
+ * fp32.  out (2D, the fp32 results widened to double) holds the normalised
+ * values before storage rounding; norms[m] fp32 (widened).  acc_out
+ * (optional, 2D int64, interleaved re/im) returns the bin sums. */
 void rmo_rope_fixed(const uint8_t *codes, int k, int s, int m, int c, int normalize,
                     const int32_t *tab, double *out, double *norms, int64_t *acc_out) {
     int nb4 = 1 << (2 * c);
@@ -266,37 +273,41 @@ void rmo_rope_fixed(const uint8_t *codes, int k, int s, int m, int c, int normal
     for (int idm = 0; idm < m; ++idm) { ar[g * m + idm] = 0; ai[g * m + idm] = 0; }
     if (acc_out)
         for (int b = 0; b < D; ++b) { acc_out[2 * b] = ar[b]; acc_out[2 * b + 1] = ai[b]; }
-    double *x = (double *)malloc(sizeof(double) * (size_t)D);
-    double *y = (double *)malloc(sizeof(double) * (size_t)D);
-    double *e = (double *)malloc(sizeof(double) * (size_t)D);
+    float *x = (float *)malloc(sizeof(float) * (size_t)D);
+    float *y = (float *)malloc(sizeof(float) * (size_t)D);
+    float *e = (float *)malloc(sizeof(float) * (size_t)D);
+    const float inv = 1.0f / 1073741824.0f; /* 2^-30 */
     for (int b = 0; b < D; ++b) {
-        x[b] = (double)ar[b] * (1.0 / RMO_S30);
-        y[b] = (double)ai[b] * (1.0 / RMO_S30);
+        x[b] = (float)ar[b] * inv; /* int64 -> fp32: correctly rounded (RNE) */
+        y[b] = (float)ai[b] * inv;
     }
-    const double eps = 2.220446049250313e-16;
+    const float eps = 2.220446049250313e-16f; /* 2^-52, exact in fp32 */
     if (normalize == 2 || normalize == 3) {
         for (int b = 0; b < D; ++b) {
-            double t1 = x[b] * x[b], t2 = y[b] * y[b];
-            double a = sqrt(t1 + t2);
-            double d = (normalize == 2) ? (a + eps) : sqrt(a + eps);
+            float t1 = x[b] * x[b], t2 = y[b] * y[b];
+            float a = sqrtf(t1 + t2);
+            float d = (normalize == 2) ? (a + eps) : sqrtf(a + eps);
             x[b] = x[b] / d; y[b] = y[b] / d;
         }
     }
-    for (int b = 0; b < D; ++b) { double t1 = x[b] * x[b], t2 = y[b] * y[b]; e[b] = t1 + t2; }
-    double cp[64];
-    for (int idm = 0; idm < m; ++idm) cp[idm] = rmo_tree_sum(e, D, m, idm);
+    for (int b = 0; b < D; ++b) { float t1 = x[b] * x[b], t2 = y[b] * y[b]; e[b] = t1 + t2; }
+    float cp[64];
+    for (int idm = 0; idm < m; ++idm) cp[idm] = rmo_tree_sum_f32(e, D, m, idm);
     if (normalize == 0) {
-        double tot = cp[0];
+        float tot = cp[0];
         for (int idm = 1; idm < m; ++idm) tot = tot + cp[idm];
-        double nrm = sqrt(tot) + 1e-16;
+        float nrm = sqrtf(tot) + 1e-16f;
         norms[0] = nrm;
         for (int idm = 1; idm < m; ++idm) norms[idm] = cp[idm];
         for (int b = 0; b < D; ++b) { out[b] = x[b] / nrm; out[D + b] = y[b] / nrm; }
     } else {
-        double add = (normalize == 1) ? 1e-7 : eps;
-        double nr[64];
-        for (int idm = 0; idm < m; ++idm) { nr[idm] = sqrt(cp[idm]) + add; norms[idm] = nr[idm]; }
-        for (int b = 0; b < D; ++b) { out[b] = x[b] / nr[b % m]; out[D + b] = y[b] / nr[b % m]; }
+        float add = (normalize == 1) ? 1e-7f : eps;
+        float nr[64];
+        for (int idm = 0; idm < m; ++idm) { nr[idm] = sqrtf(cp[idm]) + add; norms[idm] = nr[idm]; }
+        for (int b = 0; b < D; ++b) {
+            out[b] = x[b] / nr[b % m];
+            out[D + b] = y[b] / nr[b % m];
+        }
     }
     free(ar); free(ai); free(x); free(y); free(e);
 }

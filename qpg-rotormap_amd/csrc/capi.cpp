@@ -702,11 +702,14 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
         }
         if (q32 && (rc = launch_dequantize((float *)q32, (const uint8_t *)q, M * rdim, st))) return rc;
         // time each candidate on a prefix of the index sized to ~4e12 FLOP per
-        // launch (a few ms), at least 256 tiles: the plan depends on the row
+        // launch (a few ms) and at least 1 GiB, so that it streams from HBM
+        // as the whole index does (a prefix inside the 256 MB MALL favours
+        // plans that re-read the index more): the plan depends on the row
         // count, while the per-column work only scales the time (small
         // batches, HBM-bound, run over the whole index)
         rm_index sub = *idx;
-        const int64_t ncap = std::max<int64_t>(256LL * kBI, (int64_t)(4e12 / (2.0 * M * rdim)));
+        const int64_t ncap = std::max<int64_t>(cdiv(1LL << 30, rdim * esz),
+                                               (int64_t)(4e12 / (2.0 * M * rdim)));
         sub.N = std::min<int64_t>(idx->N, ncap);
         const int64_t stiles = cdiv(sub.N, kBI);
         // tiles: every ring tile up to the first that holds the batch; splits
@@ -715,6 +718,7 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
         for (int bq : {64, 128, 256}) {
             if (!tile_rows_ok(kp, bq)) continue;
             if (bq > 64 && bq / 2 >= M) continue;
+            if (bq == 64 && M > 1024) continue;  // MFMA-bound there: 4x the index passes of 256
             const int64_t R = cdiv(M, bq);
             const int64_t wave = std::max<int64_t>(1, ctx->num_cus / R);
             for (int S = 1; S <= 256 && S <= stiles; S *= 2) {

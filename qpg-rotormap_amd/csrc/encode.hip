@@ -11,14 +11,18 @@
 //   output [Re(b_0..b_{D-1}) ; Im(...)], D = m*4^c.
 //
 // MI355X design: one workgroup (4 wave64) per fragment; the fragment's words
-// are staged in LDS; each window adds an exact integer phasor (2^30 fixed
-// point, from a per-(k,s,m) table in HBM/L2) into int64 LDS bins with
-// ds_add_u64.  Integer addition is associative, so the histogram -- and
+// are staged in LDS and each lane unpacks its window's s-mer from two staged
+// words; each window adds an exact integer phasor (2^30 fixed point, from a
+// per-(k,s,m) table in HBM/L2) into int64 LDS bins with ds_add_u64, and a
+// wave whose 64 windows share one key (a ballot) reduces across lanes and
+// adds once.  Integer addition is associative, so the histogram -- and
 // everything computed from it in a fixed order afterwards -- is bitwise
 // deterministic regardless of wave scheduling (the reference's fp32 LDS
-// atomics are not).  Normalisation runs in FP64 with one rounding per
-// operation in a fixed reduction order (wave 0: 64 lane partials + halving
-// tree), so outputs are reproducible bit for bit on the CPU.
+// atomics are not).  Normalisation runs in FP32, as the reference kernel's
+// (RotorMap/encode/kernel.jl:189-279), with one correctly rounded operation
+// at a time in a fixed reduction order (wave 0: 64 lane partials + halving
+// tree), so outputs are reproducible bit for bit on the CPU; no step uses
+// FP64.
 #include "rm_device.h"
 #include "rm_internal.h"
 
@@ -26,7 +30,8 @@
 
 namespace rm {
 
-constexpr double kInvS30 = 1.0 / 1073741824.0;
+constexpr float kInvS30f = 1.0f / 1073741824.0f;  // 2^-30
+constexpr float kEps52 = 2.220446049250313e-16f;    // 2^-52 (Julia's eps()), exact in fp32
 
 // Host: the exact phasor table (int32 pairs, idm-major).  The phase fraction
 // f*p/k is formed exactly in integers before the FP64 cos/sin.
@@ -44,11 +49,11 @@ void build_phase_table(int k, int s, int m, int32_t *tab) {
     }
 }
 
-__device__ __forceinline__ double wave_tree_sum(double v) {
+__device__ __forceinline__ float wave_tree_sum(float v) {
     // lanes l < off add lane l+off; lane 0 ends with the halving-tree sum
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
-        double o = __shfl_down(v, off, 64);
+        float o = __shfl_down(v, off, 64);
         v = v + o;
     }
     return v;
@@ -68,8 +73,8 @@ __global__ __launch_bounds__(256) void rope_encode_kernel(
     const int nw = k - s + 1;
 
     long long *acc = (long long *)smem;                  // 2D int64, interleaved re/im
-    double *cp = (double *)(acc + 2 * D);                // 64 per-copy energies
-    double *nr = cp + 64;                                // 64 per-copy norms
+    float *cp = (float *)(acc + 2 * D);                  // 64 per-copy energies
+    float *nr = cp + 64;                                 // 64 per-copy norms
     uint32_t *wsh = (uint32_t *)(nr + 64);               // W + 1 words
     int32_t *cnt = (int32_t *)(wsh + W + 2);             // 4^c counts (when requested)
 
@@ -93,6 +98,32 @@ __global__ __launch_bounds__(256) void rope_encode_kernel(
         const uint32_t lower = smer & cbmask;
         const uint32_t upper = (c >= 16) ? 0u : (smer >> csh);
         const uint32_t key = lower ^ ((upper * 0x9E3779B1u) & cbmask);
+        // A ballot asks whether all 64 windows of the wave share one key (runs
+        // of N, which read as G, and other homopolymer stretches): then the
+        // wave sums its phasors across lanes and one lane adds the sums, where
+        // 64 atomics to one LDS address would serialise.  Integer sums: the
+        // result is the same either way.
+        const bool uni = __ballot(1) == ~0ull &&
+                         __ballot(key == (uint32_t)__builtin_amdgcn_readfirstlane(key)) == ~0ull;
+        if (uni) {
+            const int lane = t & 63;
+            if (counts && lane == 0) atomicAdd(&cnt[key], 64);
+            for (int idm = 0; idm < m; ++idm) {
+                const int2 ph = table[(int64_t)idm * nw + p];
+                long long x = ph.x, y = ph.y;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    x += __shfl_xor(x, off, 64);
+                    y += __shfl_xor(y, off, 64);
+                }
+                if (lane == 0) {
+                    const int b = (int)key * m + idm;
+                    atomicAdd((unsigned long long *)&acc[2 * b], (unsigned long long)x);
+                    atomicAdd((unsigned long long *)&acc[2 * b + 1], (unsigned long long)y);
+                }
+            }
+            continue;
+        }
         if (counts) atomicAdd(&cnt[key], 1);
         for (int idm = 0; idm < m; ++idm) {
             const int2 ph = table[(int64_t)idm * nw + p];
@@ -112,31 +143,32 @@ __global__ __launch_bounds__(256) void rope_encode_kernel(
     if (t < m) { acc[2 * (gkey * m + t)] = 0; acc[2 * (gkey * m + t) + 1] = 0; }
     __syncthreads();
 
-    // fixed point -> FP64 in place (the same 8 bytes per slot)
-    double *h = (double *)acc;
+    // fixed point -> fp32 in place: bin b's (re, im) go to the low 4 bytes of
+    // its own two 8-byte slots (no other thread reads them), each the
+    // correctly rounded int64 -> fp32 conversion scaled by 2^-30 (exact)
+    float *h = (float *)acc;  // re of bin b at h[4b], im at h[4b + 2]
     for (int b = t; b < D; b += 256) {
-        double x = (double)acc[2 * b] * kInvS30;
-        double y = (double)acc[2 * b + 1] * kInvS30;
+        float x = (float)acc[2 * b] * kInvS30f;
+        float y = (float)acc[2 * b + 1] * kInvS30f;
         if (normalize == 2 || normalize == 3) {
-            double t1 = x * x, t2 = y * y;
-            double a = __builtin_sqrt(t1 + t2);
-            double d = (normalize == 2) ? (a + 2.220446049250313e-16)
-                                        : __builtin_sqrt(a + 2.220446049250313e-16);
+            const float t1 = x * x, t2 = y * y;
+            const float a = __builtin_sqrtf(t1 + t2);
+            const float d = (normalize == 2) ? (a + kEps52) : __builtin_sqrtf(a + kEps52);
             x = x / d;
             y = y / d;
         }
-        h[2 * b] = x;
-        h[2 * b + 1] = y;
+        h[4 * b] = x;
+        h[4 * b + 2] = y;
     }
     __syncthreads();
 
     if (t < 64) {  // wave 0: per-copy energies in the fixed order
         for (int idm = 0; idm < m; ++idm) {
-            double part = 0.0;
+            float part = 0.0f;
             for (int j = t; idm + (int64_t)j * m < D; j += 64) {
                 const int b = idm + j * m;
-                double x = h[2 * b], y = h[2 * b + 1];
-                double t1 = x * x, t2 = y * y;
+                const float x = h[4 * b], y = h[4 * b + 2];
+                const float t1 = x * x, t2 = y * y;
                 part = part + (t1 + t2);
             }
             part = wave_tree_sum(part);
@@ -145,19 +177,19 @@ __global__ __launch_bounds__(256) void rope_encode_kernel(
         if (t == 0) {
             float *nf = norms ? norms + f * m : nullptr;
             if (normalize == 0) {
-                double tot = cp[0];
+                float tot = cp[0];
                 for (int idm = 1; idm < m; ++idm) tot = tot + cp[idm];
-                double nrm = __builtin_sqrt(tot) + 1e-16;
+                const float nrm = __builtin_sqrtf(tot) + 1e-16f;
                 for (int idm = 0; idm < m; ++idm) nr[idm] = nrm;
                 if (nf) {
-                    nf[0] = (float)nrm;
-                    for (int idm = 1; idm < m; ++idm) nf[idm] = (float)cp[idm];
+                    nf[0] = nrm;
+                    for (int idm = 1; idm < m; ++idm) nf[idm] = cp[idm];
                 }
             } else {
-                double add = (normalize == 1) ? 1e-7 : 2.220446049250313e-16;
+                const float add = (normalize == 1) ? 1e-7f : kEps52;
                 for (int idm = 0; idm < m; ++idm) {
-                    nr[idm] = __builtin_sqrt(cp[idm]) + add;
-                    if (nf) nf[idm] = (float)nr[idm];
+                    nr[idm] = __builtin_sqrtf(cp[idm]) + add;
+                    if (nf) nf[idm] = nr[idm];
                 }
             }
         }
@@ -165,9 +197,9 @@ __global__ __launch_bounds__(256) void rope_encode_kernel(
     __syncthreads();
 
     for (int b = t; b < D; b += 256) {
-        const double nv = nr[b % m];
-        const float vr = (float)(h[2 * b] / nv);
-        const float vi = (float)(h[2 * b + 1] / nv);
+        const float nv = nr[b % m];
+        const float vr = h[4 * b] / nv;
+        const float vi = h[4 * b + 2] / nv;
         store_elem(o1.p, o1.dtype, f * o1.sf + (int64_t)b * o1.sd, vr);
         store_elem(o1.p, o1.dtype, f * o1.sf + (int64_t)(b + D) * o1.sd, vi);
         if (o2.p) {
@@ -183,7 +215,7 @@ int launch_encode(const uint32_t *words, int64_t n, int k, int s, int m, int c, 
     if (n == 0) return RM_OK;
     const int D = m * (1 << (2 * c));
     const int W = (k + 15) / 16;
-    size_t lds = (size_t)2 * D * 8 + 128 * 8 + (size_t)(W + 2) * 4 + (counts ? (size_t)(1 << (2 * c)) * 4 : 0);
+    size_t lds = (size_t)2 * D * 8 + 128 * 4 + (size_t)(W + 2) * 4 + (counts ? (size_t)(1 << (2 * c)) * 4 : 0);
     RM_CHECK(lds <= 160 * 1024, RM_ERR_SHAPE,
              "encoder needs %zu B of LDS (> 160 KiB): k = %d, D = %d too large", lds, k, D);
     RM_HIP(hipFuncSetAttribute((const void *)rope_encode_kernel,

@@ -13,6 +13,14 @@ import oracle.oracle as O
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
+# The fixed-point encoder (the GPU kernel's specification) normalises in fp32:
+# against the FP64 golden its unit-energy outputs agree within 4 ulp of 1.0
+# elementwise (2^-22; about 1.7 ulp measured), its norms within 2^-21
+# relative, beyond the fixed-point table's own bound of nw * 2^-30 per bin
+# component (before normalisation).
+FP32_ATOL = 2.0 ** -22
+FP32_NORM_RTOL = 2.0 ** -21
+
 
 # ---- storage formats ------------------------------------------------------
 
@@ -97,9 +105,11 @@ def test_all_A_fragment_single_bin(s, m, c, k):
         exp[D:D + m] = h.imag / nrm
         # fixed point: every phasor is rounded to 2^-30, so the bin sums are
         # within nw * 2^-30 per component of the exact series
-        bound = 1e-9 if enc is O.ref_rope_frag_real else 2 * nw * 2.0 ** -30
-        np.testing.assert_allclose(out, exp, rtol=0, atol=max(1e-9, bound / nrm))
-        assert abs(nr[0] - nrm) <= max(1e-9 * nrm, bound)
+        fixed = enc is O.rope_fixed
+        bound = 2 * nw * 2.0 ** -30 if fixed else 1e-9
+        atol = max(1e-9, bound / nrm) + (FP32_ATOL if fixed else 0.0)
+        np.testing.assert_allclose(out, exp, rtol=0, atol=atol)
+        assert abs(nr[0] - nrm) <= max(1e-9 * nrm, bound) + (FP32_NORM_RTOL * nrm if fixed else 0)
 
 
 def test_all_G_fragment():
@@ -118,7 +128,7 @@ def test_all_G_fragment():
         nz = np.nonzero(out)[0]
         D = 4 * 256
         assert set(nz.tolist()) <= set(range(key * 4, key * 4 + 4)) | set(range(D + key * 4, D + key * 4 + 4))
-        assert abs(np.sum(out ** 2) - 1) < 1e-12
+        assert abs(np.sum(out ** 2) - 1) < (1e-6 if enc is O.rope_fixed else 1e-12)
 
 
 @pytest.mark.parametrize("s,m,c", [(8, 4, 4), (5, 1, 5)])
@@ -130,7 +140,7 @@ def test_counts_sum_and_unit_energy(s, m, c):
     assert cnt.sum() == k - s + 1
     for enc in (O.ref_rope_frag_real, O.rope_fixed):
         out, _ = enc(codes, k, s, m, c, 0)
-        assert abs(np.sum(out ** 2) - 1.0) < 1e-12
+        assert abs(np.sum(out ** 2) - 1.0) < (1e-6 if enc is O.rope_fixed else 1e-12)
 
 
 def test_histogram_phase_sum_closed_form():
@@ -179,7 +189,7 @@ def test_shift_and_mutation_sweep_anchors():
         assert abs(np.dot(a, b) - ip.real) < 1e-12
         assert abs(ip.real / abs(ip) - 0.98745 / 0.99979) < 2e-3
         assert abs(ip.real / abs(ip) - math.cos(2 * math.pi * d / k)) < 2e-3
-        assert abs(np.dot(a, a) - 1.0) < 1e-12
+        assert abs(np.dot(a, a) - 1.0) < (1e-6 if enc is O.rope_fixed else 1e-12)
     # ten edits on the pristine fragment: sub/ins/del model of the library
     import rotormap_amd as R
     from rotormap_amd import _lib as L
@@ -204,11 +214,10 @@ def test_fixed_point_within_tolerance_of_golden(s, m, c, normalize):
         codes = rng.integers(0, 4, k).astype(np.uint8)
         a, na = O.ref_rope_frag_real(codes, k, s, m, c, normalize)
         b, nb = O.rope_fixed(codes, k, s, m, c, normalize)
-        tol = 1e-8 if normalize in (0, 1) else 1e-6
-        np.testing.assert_allclose(b, a, rtol=0, atol=tol)
+        np.testing.assert_allclose(b, a, rtol=0, atol=FP32_ATOL)
         rows = [0] if normalize == 0 else range(m)
         for i in rows:
-            assert abs(nb[i] - na[i]) <= 1e-8 * max(1.0, na[i])
+            assert abs(nb[i] - na[i]) <= FP32_NORM_RTOL * max(1e-6, na[i])
 
 
 def test_golden_encoder_vs_bruteforce_python():
@@ -246,7 +255,8 @@ def test_golden_fixture_file():
         codes = np.array(cs["codes"], np.uint8) if "codes" in cs else np.full(k, cs["fill"], np.uint8)
         for enc in (O.ref_rope_frag_real, O.rope_fixed):
             out, nr = enc(codes, k, s, m, c, 0)
-            tol = 1e-9 if enc is O.ref_rope_frag_real else max(1e-9, 2 * k * 2.0 ** -30 / nr[0])
+            tol = 1e-9 if enc is O.ref_rope_frag_real else \
+                max(1e-9, 2 * k * 2.0 ** -30 / nr[0]) + FP32_ATOL
             for idx, val in cs["nonzero"]:
                 assert abs(out[idx] - val) < tol, (cs["name"], idx)
             mask = np.ones(len(out), bool)
