@@ -131,6 +131,14 @@ int rm_rowtopk_merge(rm_ctx *ctx, float *D_val, int32_t *D_loc, int64_t ldd, con
 #define RM_INDEX_FP16 2
 int rm_index_create(rm_ctx *ctx, const uint8_t *B, int64_t rdim, int64_t N, int flags,
                     rm_index **out);
+/* The classic fp16 index ("indexreal.v1"): B16 rdim x N binary16,
+ * column-major like B, kept verbatim as the RM_MODE_FP16 database -- the
+ * reference-accuracy pipeline's index_to_f16 (RotorMap/index/load.jl:96-116,
+ * RotorMap/legacy/test/e2ehuman16.jl:4-16) -- with the e4m3 database of the
+ * same columns, e4m3(fp32(fp16)), quantised on the device for the FP8 modes
+ * (index_to_f8, RotorMap/index/load.jl:76-89).  flags: RM_INDEX_HOST. */
+int rm_index_create_f16(rm_ctx *ctx, const uint16_t *B16, int64_t rdim, int64_t N, int flags,
+                        rm_index **out);
 int rm_index_destroy(rm_index *idx);
 int64_t rm_index_cols(const rm_index *idx);
 int64_t rm_index_rdim(const rm_index *idx);

@@ -489,6 +489,44 @@ int rm_index_create(rm_ctx *ctx, const uint8_t *B, int64_t rdim, int64_t N, int 
     return RM_OK;
 }
 
+int rm_index_create_f16(rm_ctx *ctx, const uint16_t *B16, int64_t rdim, int64_t N, int flags,
+                        rm_index **out) {
+    RM_CHECK(ctx && B16 && out, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(rdim > 0 && rdim % 64 == 0, RM_ERR_SHAPE, "rdim = %lld must be a positive multiple of 64",
+             (long long)rdim);
+    RM_CHECK(N >= 1 && N < ((int64_t)1 << 31) - 1, RM_ERR_SHAPE, "N = %lld out of range",
+             (long long)N);
+    RM_CHECK((flags & ~RM_INDEX_HOST) == 0, RM_ERR_ARG, "flags: only RM_INDEX_HOST applies");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    RM_HIP(hipSetDevice(ctx->device));
+    rm_index *ix = new rm_index();
+    ix->ctx = ctx;
+    ix->rdim = rdim;
+    ix->N = N;
+    const size_t n = (size_t)rdim * N;
+    if (hipMalloc(&ix->b16, n * 2) != hipSuccess || hipMalloc(&ix->b8, n) != hipSuccess) {
+        if (ix->b16) hipFree(ix->b16);
+        delete ix;
+        return fail(RM_ERR_NOMEM, "fp16 index: hipMalloc(%zu + %zu) failed", n * 2, n);
+    }
+    hipError_t e = hipMemcpyAsync(ix->b16, B16, n * 2,
+                                  (flags & RM_INDEX_HOST) ? hipMemcpyHostToDevice
+                                                          : hipMemcpyDeviceToDevice,
+                                  ctx->stream);
+    // the e4m3 database of the same columns: e4m3(fp32(fp16)), the
+    // quantisation index_to_f8 applies (RotorMap/index/load.jl:76-89)
+    int rc = e == hipSuccess ? launch_quantize(ix->b8, ix->b16, RM_F16, (int64_t)n, ctx->stream) : 0;
+    if (e == hipSuccess && rc == RM_OK) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess || rc != RM_OK) {
+        hipFree(ix->b8);
+        hipFree(ix->b16);
+        delete ix;
+        return rc ? rc : fail(RM_ERR_HIP, "fp16 index upload failed: %s", hipGetErrorString(e));
+    }
+    *out = ix;
+    return RM_OK;
+}
+
 int rm_index_destroy(rm_index *idx) {
     if (!idx) return RM_OK;
     hipSetDevice(idx->ctx->device);
@@ -718,7 +756,10 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
         for (int bq : {64, 128, 256}) {
             if (!tile_rows_ok(kp, bq)) continue;
             if (bq > 64 && bq / 2 >= M) continue;
-            if (bq == 64 && M > 1024) continue;  // MFMA-bound there: 4x the index passes of 256
+            // MFMA-bound from a few hundred rows: a small tile re-reads the index
+            // (and runs more epilogue work per MFMA) without saving padding
+            if (bq == 64 && M > 1024) continue;
+            if (bq == 128 && M > 16384) continue;
             const int64_t R = cdiv(M, bq);
             const int64_t wave = std::max<int64_t>(1, ctx->num_cus / R);
             for (int S = 1; S <= 256 && S <= stiles; S *= 2) {

@@ -51,6 +51,7 @@ SIGNATURES = {
     "rm_gemm_chunk": (i32, [vp, vp, i32, i64, vp, i64, i64, vp, i64, i32, i64, i64, i64, vp]),
     "rm_rowtopk_merge": (i32, [vp, vp, vp, i64, vp, i32, i64, i64, i64, i64, i32, vp]),
     "rm_index_create": (i32, [vp, vp, i64, i64, i32, C.POINTER(vp)]),
+    "rm_index_create_f16": (i32, [vp, vp, i64, i64, i32, C.POINTER(vp)]),
     "rm_index_destroy": (i32, [vp]),
     "rm_index_cols": (i64, [vp]),
     "rm_index_rdim": (i64, [vp]),

@@ -40,6 +40,9 @@ class IndexData:
     format: str = FP8_FORMAT
     rec_of_col: np.ndarray = None   # 0-based record index per column
     rec_names: list = field(default_factory=list)
+    # the classic fp16 rows ("indexreal.v1" embeds, (N, rdim)) when built with
+    # keep_f16: the FP16 path's verbatim database (RotorMap/index/load.jl:96-116)
+    embeds16: np.ndarray = None
 
     @property
     def encoder(self):
@@ -60,8 +63,11 @@ def window_table(ref, k, kstep):
     return np.concatenate(recs), np.concatenate(starts)
 
 
-def build_index(ref, re, kstep=None, batch_size=None, device=0, normalize=0, chunk=1 << 16):
-    """Encode every window of `ref` (both strands) on the GPU into e4m3 columns."""
+def build_index(ref, re, kstep=None, batch_size=None, device=0, normalize=0, chunk=1 << 16,
+                keep_f16=False):
+    """Encode every window of `ref` (both strands) on the GPU into e4m3 columns
+    (and, keep_f16, the fp16 rows of the classic layout from the same encoder
+    pass: e4m3 = e4m3(fp32(fp16)) of them)."""
     import torch
     k = re.k
     kstep = kstep or max(k // 10, 1)
@@ -82,6 +88,7 @@ def build_index(ref, re, kstep=None, batch_size=None, device=0, normalize=0, chu
     N = 2 * nwin
     dev = f"cuda:{device}"
     emb = torch.empty(N, rd, dtype=torch.uint8, device=dev)
+    emb16 = torch.empty(N, rd, dtype=torch.float16, device=dev) if keep_f16 else None
     norms = torch.empty(N, re.m, dtype=torch.float32, device=dev)
     recw = [torch.from_numpy(r.words.view(np.int32)).to(dev) for r in ref.records]
     for lo in range(0, nwin, chunk):
@@ -100,10 +107,13 @@ def build_index(ref, re, kstep=None, batch_size=None, device=0, normalize=0, chu
         revcomp_frag_words(rc, fw, k, n)
         for words, cols in ((fw, col_f[lo:hi]), (rc, col_r[lo:hi])):
             e = torch.empty(n, rd, dtype=torch.uint8, device=dev)
+            e16 = torch.empty(n, rd, dtype=torch.float16, device=dev) if keep_f16 else None
             nn = torch.empty(n, re.m, dtype=torch.float32, device=dev)
-            encode_frag_real_batch(e, nn.t(), re, words, normalize)
+            encode_frag_real_batch(e, nn.t(), re, words, normalize, dest2=e16)
             ci = torch.from_numpy(cols).to(dev)
             emb[ci] = e
+            if keep_f16:
+                emb16[ci] = e16
             norms[ci] = nn
     torch.cuda.synchronize(device)
     names = [r.name for r in ref.records]
@@ -118,7 +128,8 @@ def build_index(ref, re, kstep=None, batch_size=None, device=0, normalize=0, chu
     strand[col_r] = 1
     heads = [ref.records[j].header for j in rec_of_col.tolist()]
     return IndexData(emb.cpu().numpy(), norms.cpu().numpy(), heads, starts, strand, k, kstep,
-                     re.s, re.m, re.c, normalize, nwin, "", FP8_FORMAT, rec_of_col, names)
+                     re.s, re.m, re.c, normalize, nwin, "", FP8_FORMAT, rec_of_col, names,
+                     emb16.cpu().numpy() if keep_f16 else None)
 
 
 def save_index(d, path):

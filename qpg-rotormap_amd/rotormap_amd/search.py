@@ -31,16 +31,27 @@ class ResidentIndex:
     RotorMap/index/load.jl:130-143) for the FP16 path."""
 
     def __init__(self, B, fp16=False, device=0):
+        """B (N, rdim): uint8 e4m3 rows, or float16 rows -- the classic fp16
+        index kept verbatim for the FP16 mode (rm_index_create_f16), with its
+        e4m3 quantisation for the FP8 modes."""
         self.device = device
         N, rd = B.shape
-        flags = L.RM_INDEX_FP16 if fp16 else 0
+        verbatim = str(B.dtype) in ("float16", "torch.float16")
+        flags = L.RM_INDEX_FP16 if (fp16 and not verbatim) else 0
         if isinstance(B, np.ndarray):
-            B = np.ascontiguousarray(B, dtype=np.uint8)
+            B = np.ascontiguousarray(B)
             flags |= L.RM_INDEX_HOST
         h = C.c_void_p()
-        L.check(L.lib().rm_index_create(L.ctx(device), L.ptr(B), rd, N, flags, C.byref(h)))
+        if verbatim:
+            L.check(L.lib().rm_index_create_f16(L.ctx(device), L.ptr(B), rd, N, flags, C.byref(h)))
+        else:
+            if str(B.dtype) not in ("uint8", "torch.uint8"):
+                raise L.RotorMapArgumentError(L.RM_ERR_ARG, f"index rows must be uint8 e4m3 or float16, not {B.dtype}")
+            L.check(L.lib().rm_index_create(L.ctx(device), L.ptr(B), rd, N, flags, C.byref(h)))
         self.handle = h
-        self.cols, self.rdim, self.fp16 = int(N), int(rd), bool(fp16)
+        self.cols, self.rdim = int(N), int(rd)
+        self.fp16 = bool(fp16 or verbatim)
+        self.verbatim_f16 = verbatim
 
     def close(self):
         if self.handle:

@@ -18,7 +18,7 @@ K, KSTEP = 20_000, 2_000
 def tiny(torch_gpu):
     ref = R.synth_reference([600_000, 400_000], gc=0.5, seed=2024)
     re = R.RopeEncoder(k=K, s=8, m=4, c=4)
-    idx = R.build_index(ref, re, kstep=KSTEP)
+    idx = R.build_index(ref, re, kstep=KSTEP, keep_f16=True)
     assert idx.embeds8.shape == (964, 2048)
     exact = R.sample_reads(ref, 1000, K, err=0.0, seed=42, draw="random")
     mut = R.sample_reads(ref, 200, K, err=0.05, seed=43, draw="random", model=0)
@@ -89,3 +89,20 @@ def test_alone_equals_in_mix(torch_gpu, tiny):
         v_one, l_one = run(torch_gpu, idx, re, sub, mode)
         np.testing.assert_array_equal(v_one, v_all[5:9])
         np.testing.assert_array_equal(l_one, l_all[5:9])
+
+
+def test_tiny_fp16_verbatim_index(torch_gpu, tiny):
+    """The classic fp16 index from the same encoder pass: its e4m3
+    quantisation is the e4m3 index byte for byte, and the FP16 mode over the
+    verbatim fp16 columns meets the same recall floors."""
+    ref, re, idx, exact, mut = tiny
+    assert idx.embeds16.dtype == np.float16 and idx.embeds16.shape == idx.embeds8.shape
+    np.testing.assert_array_equal(O.f32_to_e4m3(idx.embeds16.astype(np.float32).ravel()),
+                                  idx.embeds8.ravel())
+    B = R.ResidentIndex(idx.embeds16)
+    for reads, floor in ((exact, 1000), (mut, 180)):
+        eng = R.TopKEngine(B, k=20, mode="fp16")
+        dw = torch_gpu.from_numpy(reads.words.view(np.int32).reshape(-1).copy()).cuda()
+        v, l = eng.map_words(re, dw, reads.words.shape[0])
+        torch_gpu.cuda.synchronize()
+        assert score(idx, reads, l.cpu().numpy())["correct"] >= floor

@@ -380,3 +380,40 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
         assert plan(4_096, L.RM_MODE_FP8) == expect(4_096, 2048)
     finally:
         idx.close()
+
+
+def test_fp16_verbatim_index(torch_gpu):
+    """rm_index_create_f16 keeps the fp16 columns verbatim for the FP16 mode
+    (index_to_f16) and quantises them to e4m3(fp32(fp16)) on the device for
+    the FP8 modes (index_to_f8): the device e4m3 copy equals the host
+    quantisation, and the fused FP16 top-k over the verbatim columns equals
+    the oracle's top-k of the GPU's own fp16 C, for every tile shape."""
+    torch = torch_gpu
+    rng = np.random.default_rng(123)
+    M, N, K = 300, 1500, 512
+    Q = (rng.standard_normal((M, K)) * 0.05).astype(np.float16)
+    B16 = (rng.standard_normal((N, K)) * 0.05).astype(np.float16)
+    idx = R.ResidentIndex(B16)
+    assert idx.verbatim_f16 and idx.fp16
+    deq = torch.empty(N * K, dtype=torch.float32, device="cuda")
+    L.check(R.lib().rm_dequantize_e4m3(L.ctx(0), L.ptr(deq), R.lib().rm_index_e4m3(idx.handle),
+                                       N * K, None))
+    torch.cuda.synchronize()
+    exp8 = O.f32_to_e4m3(B16.astype(np.float32).ravel())
+    np.testing.assert_array_equal(deq.cpu().numpy(), O.e4m3_table()[exp8])
+    C = torch.zeros(N, M, dtype=torch.float32, device="cuda").t()
+    R.fp8_gemm_hip(C, torch.from_numpy(Q).cuda(), torch.from_numpy(B16).cuda(), N)
+    torch.cuda.synchronize()
+    ev, el = O.topk_rows(C.cpu().numpy(), 20)
+    eng = R.TopKEngine(idx, k=20, mode="fp16")
+    for tile in TILES:
+        v, l = fused(torch, eng, Q, splits=2, tile=tile)
+        np.testing.assert_array_equal(l, el, err_msg=f"tile={tile}")
+        np.testing.assert_array_equal(v, ev, err_msg=f"tile={tile}")
+    # the FP8 modes run on the quantised copy
+    eng8 = R.TopKEngine(idx, k=20, mode="fp8")
+    eng8h = R.TopKEngine(exp8.reshape(N, K), k=20, mode="fp8")
+    Q8 = O.f32_to_e4m3(Q.astype(np.float32).ravel()).reshape(M, K)
+    a, b = fused(torch, eng8, Q8), fused(torch, eng8h, Q8)
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[0], b[0])
