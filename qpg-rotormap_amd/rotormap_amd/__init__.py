@@ -16,6 +16,7 @@ from .index import (IndexData, build_index, load_index, load_index_dir, save_ind
                     save_index_dir, window_table)
 from .reads import ReadSet, even_alloc, parse_read_head, sample_reads
 from .serve import Server, ServedRequest, poisson_trace, run_trace
-from .search import MODES, ResidentIndex, TopKBatch, TopKEngine, count_mapped, topk_flow
+from .search import MODES, ResidentIndex, TopKBatch, TopKEngine, count_mapped
+from .stream import RopeRealBatch, rope_encode_real_stream, rope_topk_stream, topk_flow
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -2,8 +2,9 @@
 streamed flow and the recall scorer.
 
 Mirrors RotorMap/search/engine_fp8.jl (TopKEngine :290-337, batch_gemm_topk!
-:399-437, topk_flow :498-547, map_and_count :654-733) and its FP16 twin
-RotorMap/search/engine_fp16.jl; engine `:hip` is this library.
+:399-437, map_and_count :654-733) and its FP16 twin
+RotorMap/search/engine_fp16.jl; engine `:hip` is this library.  The streamed
+flows (topk_flow, rope_topk_stream) are in stream.py.
 """
 import ctypes as C
 from dataclasses import dataclass, field
@@ -162,23 +163,6 @@ class TopKEngine:
         """Pin (or with all zeros clear) the plan of M's size bucket."""
         L.check(L.lib().rm_plan_set(L.ctx(self.device), int(M), self.mode, int(tile_rows),
                                     int(splits), int(qgroup)))
-
-
-def topk_flow(source, eng, re):
-    """Consume batches (words (nb, W) uint32 host, heads, first) and yield
-    TopKBatch with host matrices -- topk_flow (RotorMap/search/engine_fp8.jl:
-    498-547) with the encoder fused in (words go straight to the device)."""
-    import torch
-    for words, heads, first in source:
-        nb = words.shape[0]
-        if nb == 0:
-            continue
-        if nb > eng.rows_cap:
-            raise L.RotorMapArgumentError(L.RM_ERR_ARG, f"batch height {nb} > engine rows_cap {eng.rows_cap}")
-        dw = torch.from_numpy(np.ascontiguousarray(words).view(np.int32)).to(f"cuda:{eng.device}")
-        norms = torch.empty(nb, re.m, dtype=torch.float32, device=f"cuda:{eng.device}").t()
-        v, l = eng.map_words(re, dw, nb, norms)
-        yield TopKBatch(v.cpu().numpy(), l.cpu().numpy(), norms.cpu().numpy(), list(heads), first)
 
 
 def count_mapped(locs, read_rec, read_start, col_rec, col_start, dbk, ktop=None):

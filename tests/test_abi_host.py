@@ -35,16 +35,33 @@ def test_every_header_function_is_exported_and_bound():
     assert not extra, f"bound but not declared: {extra}"
 
 
-def test_julia_glue_calls_declared_functions():
+JULIA_GLUE = ["RotorMapHIP.jl", "RotorMap/common/hip.jl", "RotorMap/encode/kernel_hip.jl",
+              "RotorMap/gemm/fp8_hip.jl", "RotorMap/search/engine_hip.jl"]
+
+
+@pytest.mark.parametrize("path", JULIA_GLUE)
+def test_julia_glue_calls_declared_functions(path):
     """Julia is not in this image, so the glue cannot run here: every symbol it
-    ccalls must at least be declared in the header and exported."""
-    glue = os.path.join(ROOT, "qpg-rotormap_amd", "julia", "RotorMapHIP.jl")
-    called = sorted(set(re.findall(r"ccall\(\(:(rm_\w+),", open(glue).read())))
-    assert len(called) >= 10, called
-    declared = set(header_functions())
-    assert not [n for n in called if n not in declared]
+    ccalls must at least be declared in the header and exported, with as many
+    argument types as the C declaration has parameters; blocks balance."""
+    glue = os.path.join(ROOT, "qpg-rotormap_amd", "julia", path)
+    text = open(glue).read()
+    calls = re.findall(r"ccall\(\(:(rm_\w+),\s*\w+\),\s*\w+,\s*\(([^)]*)\)", text, re.S)
+    assert calls, path
+    declared = header_functions()
     lib = C.CDLL(L.LIB_PATH)
-    assert all(hasattr(lib, n) for n in called)
+    for name, argt in calls:
+        assert name in declared, name
+        assert hasattr(lib, name), name
+        nargs = len([a for a in argt.replace("\n", " ").split(",") if a.strip()])
+        assert nargs == len(L.SIGNATURES[name][1]), (path, name, nargs)
+    # every opening block keyword has its `end` (a crude balance check)
+    code = re.sub(r'"""(?:.|\n)*?"""', "", text)
+    code = re.sub(r"#.*", "", code)
+    code = re.sub(r'"(?:[^"\\]|\\.)*"', '""', code)
+    opens = len(re.findall(r"\b(function|module|mutable struct|struct|if|for|while|try|do|begin|let)\b", code))
+    ends = len(re.findall(r"\bend\b", code)) - len(re.findall(r"\[[^\]\n]*\bend\b[^\]\n]*\]", code))
+    assert opens == ends, (path, opens, ends)
 
 
 def test_header_and_integration_example_compile_as_c(tmp_path):
