@@ -330,45 +330,82 @@ __device__ __forceinline__ void ring_wait(int wid) {
     }
 }
 
-// One stage: wait until stage s (issued kRing - 1 stages ago) has landed in
-// every wave's rows, then load stage s + kRing - 1 into the buffer stage s - 1
-// left and run stage s's MFMAs.  The restrict pointers give the fragment
-// reads an alias scope apart from the loads, so the compiler does not wait
-// for the loads in flight before each read.
-template <int ELEM, class SH, bool FIRST>
-__device__ __forceinline__ void ring_stage(v16f (&acc)[SH::AI][2], const RingSrc &d,
-                                           uint8_t *__restrict__ bw, const uint8_t *__restrict__ br,
-                                           int s, int nk, int wi, int wr, int lane) {
-    constexpr int AI = SH::AI, H = (AI + 1) / 2;
-    ring_wait<SH>(d.wid);
-    Frags<AI> f;
-    ld_frags_b<ELEM, AI>(f, br, wr, lane);
-    ld_frags_a<ELEM, AI, 0, H>(f, br, wi, lane);
-    ring_load<SH>(d, bw, min(s + kRing - 1, nk - 1) * kBK);  // past the end: an unread buffer
-    mma_part<ELEM, AI, 0, H, FIRST>(acc, f);
-    ld_frags_a<ELEM, AI, H, AI>(f, br, wi, lane);
-    mma_part<ELEM, AI, H, AI, FIRST>(acc, f);
+// Waves 4-7 of a ring tile (the second wave on each SIMD) can run one stage
+// behind waves 0-3 (RM_LAG): in the step of stage s a lagging wave first runs
+// the MFMAs of stage s-1 on fragments it read in the previous step, then
+// reads stage s's fragments; waves 0-3 read first and compute after.  Within
+// a step the two waves of a SIMD then overlap one's LDS reads with the
+// other's MFMAs, where in lockstep both read right after the barrier and the
+// MFMA pipe idles.  Same registers (the fragments live across the barrier),
+// same barriers and buffer lifetimes: every read of a stage still completes
+// before the barrier that precedes the refill of its buffer.
+#ifndef RM_LAG
+#define RM_LAG 1
+#endif
+// RM_SETPRIO: raise the wave priority while it issues a block of MFMAs
+#ifndef RM_SETPRIO
+#define RM_SETPRIO 0
+#endif
+#ifndef RM_LAG16
+#define RM_LAG16 0
+#endif
+template <int ELEM>
+constexpr bool kLag = ELEM == 0 ? RM_LAG != 0 : RM_LAG16 != 0;
+
+template <int ELEM, int AI>
+__device__ __forceinline__ void zero_acc(v16f (&acc)[AI][2]) {
+    const v16f zero = {};
+#pragma unroll
+    for (int ai = 0; ai < AI; ++ai)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[ai][bj] = zero;
 }
 
-// A stage of the streamed loop that loads stage `kb` of `d` (the next tile)
-// into bw, or (LOAD false) loads nothing.
-template <int ELEM, class SH, bool LOAD>
-__device__ __forceinline__ void ring_stage_next(v16f (&acc)[SH::AI][2], const RingSrc &d,
-                                                uint8_t *__restrict__ bw,
-                                                const uint8_t *__restrict__ br, int kb, int wi,
-                                                int wr, int lane) {
+// A lagging wave's tile tail: the MFMAs of the tile's last stage.
+template <int ELEM, class SH, bool LAG>
+__device__ __forceinline__ void ring_lag_tail(v16f (&acc)[SH::AI][2], const Frags<SH::AI> &f) {
+    if constexpr (LAG) mma_part<ELEM, SH::AI, 0, SH::AI, false>(acc, f);
+}
+
+// One stage: wait until stage s (issued kRing - 1 stages ago) has landed in
+// every wave's rows, then load stage s + kRing - 1 into the buffer stage s - 1
+// left and run stage s's MFMAs (a lagging wave: stage s-1's, then read stage
+// s).  The restrict pointers give the fragment reads an alias scope apart
+// from the loads, so the compiler does not wait for the loads in flight
+// before each read.  LOAD false: no load (the streamed loop's last stage).
+template <int ELEM, class SH, bool LAG, bool FIRST, bool LOAD = true>
+__device__ __forceinline__ void ring_stage(v16f (&acc)[SH::AI][2], Frags<SH::AI> &f,
+                                           const RingSrc &d, uint8_t *__restrict__ bw,
+                                           const uint8_t *__restrict__ br, int kb, int wi, int wr,
+                                           int lane) {
     constexpr int AI = SH::AI, H = (AI + 1) / 2;
     ring_wait<SH>(d.wid);
-    Frags<AI> f;
+    if constexpr (LAG) {
+        if constexpr (LOAD) ring_load<SH>(d, bw, kb);
+        if constexpr (FIRST) {
+            zero_acc<ELEM, AI>(acc);
+        } else {
+            if (RM_SETPRIO) __builtin_amdgcn_s_setprio(1);
+            mma_part<ELEM, AI, 0, AI, false>(acc, f);
+            if (RM_SETPRIO) __builtin_amdgcn_s_setprio(0);
+        }
+        ld_frags_b<ELEM, AI>(f, br, wr, lane);
+        ld_frags_a<ELEM, AI, 0, AI>(f, br, wi, lane);
+        return;
+    }
     ld_frags_b<ELEM, AI>(f, br, wr, lane);
     ld_frags_a<ELEM, AI, 0, H>(f, br, wi, lane);
     if constexpr (LOAD) ring_load<SH>(d, bw, kb);
-    mma_part<ELEM, AI, 0, H, false>(acc, f);
+    if (RM_SETPRIO) __builtin_amdgcn_s_setprio(1);
+    mma_part<ELEM, AI, 0, H, FIRST>(acc, f);
+    if (RM_SETPRIO) __builtin_amdgcn_s_setprio(0);
     ld_frags_a<ELEM, AI, H, AI>(f, br, wi, lane);
-    mma_part<ELEM, AI, H, AI, false>(acc, f);
+    if (RM_SETPRIO) __builtin_amdgcn_s_setprio(1);
+    mma_part<ELEM, AI, H, AI, FIRST>(acc, f);
+    if (RM_SETPRIO) __builtin_amdgcn_s_setprio(0);
 }
 
-template <int ELEM, class SH>
+template <int ELEM, class SH, bool LAG>
 __device__ __forceinline__ void tile_product_ring(v16f (&acc)[SH::AI][2], const GemmArgs &a,
                                                   int64_t i0, int64_t r0, uint8_t *ring, int tid,
                                                   int wi, int wr, int lane) {
@@ -381,24 +418,29 @@ __device__ __forceinline__ void tile_product_ring(v16f (&acc)[SH::AI][2], const 
     const int nk = a.Kb / kBK;  // >= 1
     constexpr int SB = SH::STAGE_BYTES;
     uint8_t *b0 = ring, *b1 = ring + SB, *b2 = ring + 2 * SB, *b3 = ring + 3 * SB;
+    // stage s loads stage s + kRing - 1 (past the end: the last stage again,
+    // into a buffer no later stage reads)
+    auto kl = [&](int s) { return min(s + kRing - 1, nk - 1) * kBK; };
+    Frags<SH::AI> f;
     // the callers' last barrier ordered every earlier use of the ring
     ring_load<SH>(d, b0, 0);
     ring_load<SH>(d, b1, min(1, nk - 1) * kBK);
     ring_load<SH>(d, b2, min(2, nk - 1) * kBK);
-    ring_stage<ELEM, SH, true>(acc, d, b3, b0, 0, nk, wi, wr, lane);
+    ring_stage<ELEM, SH, LAG, true>(acc, f, d, b3, b0, kl(0), wi, wr, lane);
     int s = 1;
     for (; s + 3 < nk; s += 4) {
-        ring_stage<ELEM, SH, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
-        ring_stage<ELEM, SH, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
-        ring_stage<ELEM, SH, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
-        ring_stage<ELEM, SH, false>(acc, d, b3, b0, s + 3, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, LAG, false>(acc, f, d, b0, b1, kl(s), wi, wr, lane);
+        ring_stage<ELEM, SH, LAG, false>(acc, f, d, b1, b2, kl(s + 1), wi, wr, lane);
+        ring_stage<ELEM, SH, LAG, false>(acc, f, d, b2, b3, kl(s + 2), wi, wr, lane);
+        ring_stage<ELEM, SH, LAG, false>(acc, f, d, b3, b0, kl(s + 3), wi, wr, lane);
     }
-    if (s < nk) ring_stage<ELEM, SH, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
-    if (s + 1 < nk) ring_stage<ELEM, SH, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
-    if (s + 2 < nk) ring_stage<ELEM, SH, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
+    if (s < nk) ring_stage<ELEM, SH, LAG, false>(acc, f, d, b0, b1, kl(s), wi, wr, lane);
+    if (s + 1 < nk) ring_stage<ELEM, SH, LAG, false>(acc, f, d, b1, b2, kl(s + 1), wi, wr, lane);
+    if (s + 2 < nk) ring_stage<ELEM, SH, LAG, false>(acc, f, d, b2, b3, kl(s + 2), wi, wr, lane);
     // every load of the tile (the clamped ones too) has landed and every
     // fragment read is done before the epilogue reuses the ring
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    ring_lag_tail<ELEM, SH, LAG>(acc, f);
 }
 
 // The same K loop as one stream over the tiles of an item (K a multiple of
@@ -410,7 +452,7 @@ __device__ __forceinline__ void tile_product_ring(v16f (&acc)[SH::AI][2], const 
 // in place of the next tile's (into buffers 0-1, which the next item's
 // first tile refills after the item's closing barrier) so that every stage
 // keeps two younger stages in flight for ring_wait.
-template <int ELEM, class SH>
+template <int ELEM, class SH, bool LAG>
 __device__ __forceinline__ void tile_product_ring_stream(v16f (&acc)[SH::AI][2], const GemmArgs &a,
                                                          int64_t i0, int64_t r0, uint8_t *ring,
                                                          int tid, int wi, int wr, int lane,
@@ -431,6 +473,7 @@ __device__ __forceinline__ void tile_product_ring_stream(v16f (&acc)[SH::AI][2],
     }
     constexpr int SB = SH::STAGE_BYTES;
     uint8_t *b0 = ring, *b1 = ring + SB, *b2 = ring + 2 * SB, *b3 = ring + 3 * SB;
+    Frags<SH::AI> f;
     // the callers' last barrier ordered every earlier use of buffers 2-3
     // (and of 0-1 on an item's first tile; this wave's loads into its rows of
     // 0-1 from the previous item have landed)
@@ -440,19 +483,20 @@ __device__ __forceinline__ void tile_product_ring_stream(v16f (&acc)[SH::AI][2],
         ring_load<SH>(d, b1, kBK);
     }
     ring_load<SH>(d, b2, 2 * kBK);
-    ring_stage<ELEM, SH, true>(acc, d, b3, b0, 0, nk, wi, wr, lane);
+    ring_stage<ELEM, SH, LAG, true>(acc, f, d, b3, b0, 3 * kBK, wi, wr, lane);
     for (int s = 1; s + 3 < nk; s += 4) {
-        ring_stage<ELEM, SH, false>(acc, d, b0, b1, s, nk, wi, wr, lane);
-        ring_stage<ELEM, SH, false>(acc, d, b1, b2, s + 1, nk, wi, wr, lane);
-        ring_stage<ELEM, SH, false>(acc, d, b2, b3, s + 2, nk, wi, wr, lane);
-        ring_stage<ELEM, SH, false>(acc, d, b3, b0, s + 3, nk, wi, wr, lane);
+        ring_stage<ELEM, SH, LAG, false>(acc, f, d, b0, b1, (s + 3) * kBK, wi, wr, lane);
+        ring_stage<ELEM, SH, LAG, false>(acc, f, d, b1, b2, (s + 4) * kBK, wi, wr, lane);
+        ring_stage<ELEM, SH, LAG, false>(acc, f, d, b2, b3, (s + 5) * kBK, wi, wr, lane);
+        ring_stage<ELEM, SH, LAG, false>(acc, f, d, b3, b0, (s + 6) * kBK, wi, wr, lane);
     }
     // stages nk-3 .. nk-1 (buffers 1, 2, 3) load the next tile's 0 and 1
-    ring_stage_next<ELEM, SH, true>(acc, dn, b0, b1, kn, wi, wr, lane);
-    ring_stage_next<ELEM, SH, true>(acc, dn, b1, b2, kn + (has_next ? kBK : 0), wi, wr, lane);
-    ring_stage_next<ELEM, SH, false>(acc, dn, b2, b3, 0, wi, wr, lane);
+    ring_stage<ELEM, SH, LAG, false>(acc, f, dn, b0, b1, kn, wi, wr, lane);
+    ring_stage<ELEM, SH, LAG, false>(acc, f, dn, b1, b2, kn + (has_next ? kBK : 0), wi, wr, lane);
+    ring_stage<ELEM, SH, LAG, false, false>(acc, f, dn, b2, b3, 0, wi, wr, lane);
     // every fragment read is done before the epilogue uses buffers 2-3
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    ring_lag_tail<ELEM, SH, LAG>(acc, f);
 }
 
 // Float <-> int key with the same order (signed int compare): negative
@@ -800,11 +844,22 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
         if constexpr (RING) {
-            if ((a.Kb / kBK) % 4 == 0)
-                tile_product_ring_stream<ELEM, SH>(acc, a, i0, r0, ring, tid, wi, wr, lane, t == t0,
-                                                   t + 1 < t1);
-            else
-                tile_product_ring<ELEM, SH>(acc, a, i0, r0, ring, tid, wi, wr, lane);
+            // waves 4-7 run one stage behind (RM_LAG): a separate copy of the
+            // K loop per role, a wave-uniform branch
+            const bool lag = kLag<ELEM> && __builtin_amdgcn_readfirstlane(tid >> 6) >= 4;
+            if ((a.Kb / kBK) % 4 == 0) {
+                if (lag)
+                    tile_product_ring_stream<ELEM, SH, kLag<ELEM>>(acc, a, i0, r0, ring, tid, wi, wr,
+                                                                    lane, t == t0, t + 1 < t1);
+                else
+                    tile_product_ring_stream<ELEM, SH, false>(acc, a, i0, r0, ring, tid, wi, wr,
+                                                              lane, t == t0, t + 1 < t1);
+            } else {
+                if (lag)
+                    tile_product_ring<ELEM, SH, kLag<ELEM>>(acc, a, i0, r0, ring, tid, wi, wr, lane);
+                else
+                    tile_product_ring<ELEM, SH, false>(acc, a, i0, r0, ring, tid, wi, wr, lane);
+            }
         } else
             tile_product<ELEM, SH>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
         // columns of this tile valid for this lane: RC(ai, i) < lim
