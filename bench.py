@@ -29,6 +29,9 @@ Legs beside the headline step:
                     FP64 dots of the same operands, and against the ideal FP64
                     score (golden FP64 encoder on both sides, no quantisation)
   small             one launch of 1-4,096 reads: index GB/s against 8 TB/s
+  unfused           the reference's plug point for 8,192 reads: rm_gemm_chunk into
+                    65,536-column C chunks (fp16 or fp32) + rm_rowtopk_merge, chunk
+                    by chunk over the index, against the fused launch
   uneven (human)    the first 512 of 4,096 batch sizes log-uniform in
                     [1, 131,072] (err in [0, 0.10]): one launch per batch vs
                     one persistent launch, after the autotuner
@@ -173,6 +176,8 @@ def main():
     ap.add_argument("--no-fasta", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--no-unfused", action="store_true",
+                    help="skip timing the reference's unfused plug point (chunk GEMM + row top-k)")
     ap.add_argument("--no-vendor", action="store_true",
                     help="skip timing hipBLASLt's GEMM at the same M and K")
     ap.add_argument("--serve-rates", default=None,
@@ -441,6 +446,67 @@ def main():
                     "max(HBM, MFMA) fraction" % PEAK_HBM_GBS, "launches": small}
         log("small batches: " + ", ".join(f"{s_['reads']}: {s_['kernel_ms']} ms "
                                           f"{s_['index_gb_s']} GB/s" for s_ in small))
+
+    # ---- the reference's unfused plug point: chunk GEMM into C + row top-k ----
+    if not args.no_unfused:
+        M8, wch = min(8192, M), 65536
+        q8 = torch.empty(M8, RDIM, dtype=torch.uint8, device=dev)
+        R.encode_frag_real_batch(q8, None, re, words[:M8 * W])
+        Bt = torch.from_numpy(idx.embeds8).to(dev)
+        ev = lambda: torch.cuda.Event(enable_timing=True)
+        uf = {"reads": M8, "chunk_columns": wch, "chunks": -(-N // wch)}
+
+        def chunk_pass(cdt, topk=True):
+            C_ = torch.empty(wch, M8, dtype=cdt, device=dev).t()
+            Dv = torch.full((20, M8), float("-inf"), device=dev).t()
+            Dl = torch.zeros(20, M8, dtype=torch.int32, device=dev).t()
+            e0, e1 = ev(), ev()
+            e0.record()
+            for lo in range(0, N, wch):
+                wi = min(wch, N - lo)
+                R.fp8_gemm_hip(C_[:, :wi], q8, Bt, wi, b_col0=lo)
+                if topk:
+                    R.launch_seg_rowtopk_merge(Dv, Dl, C_[:, :wi], lo, 20)
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1), Dv.cpu().numpy(), Dl.cpu().numpy()
+
+        chunk_pass(torch.float16)  # warm-up
+        engq = R.TopKEngine(B8, k=20, mode="fp8", rows_cap=M8)
+        engq.batch_gemm_topk(q8)
+        torch.cuda.synchronize()
+        prof_read()
+        L.check(lib.rm_prof_enable(ctx, 1))
+        fv, fl = engq.batch_gemm_topk(q8)
+        torch.cuda.synchronize()
+        fk, _, _ = prof_read()
+        L.check(lib.rm_prof_enable(ctx, 0))
+        fv, fl = fv.cpu().numpy(), fl.cpu().numpy()
+        fl_ = 2.0 * M8 * RDIM * N
+        for name, cdt in (("fp16_C", torch.float16), ("fp32_C", torch.float32)):
+            t_all, dv_, dl_ = chunk_pass(cdt)
+            t_gemm, _, _ = chunk_pass(cdt, topk=False)
+            same = [set(a.tolist()) == set(b.tolist()) for a, b in zip(dl_, fl)]
+            uf[name] = {"ms": rnd(t_all, 3), "gemm_ms": rnd(t_gemm, 3), "rowtopk_ms": rnd(t_all - t_gemm, 3),
+                        "tflops": rnd(fl_ / t_all / 1e9, 1),
+                        "c_bytes_per_chunk": M8 * wch * (2 if cdt == torch.float16 else 4),
+                        "rowtopk_c_gb_s": rnd(M8 * N * (2 if cdt == torch.float16 else 4)
+                                              / max(t_all - t_gemm, 1e-6) / 1e6, 1),
+                        "same_top20_set_as_fused": rnd(np.mean(same), 5)}
+            if cdt == torch.float32:
+                uf[name]["bit_identical_to_fused"] = bool(np.array_equal(dl_, fl) and np.array_equal(dv_, fv))
+        uf["fused_ms"] = rnd(fk, 3)
+        uf["fused_tflops"] = rnd(fl_ / fk / 1e9, 1)
+        uf["note"] = ("plain FP8 (e4m3 x e4m3, fp32 accumulate, top-20); the fp32-C path ranks the "
+                      "same fp32 scores as the fused launch, the fp16-C path the reference's fp16 "
+                      "chunk (cout = :f16)")
+        result["unfused_plug_point"] = uf
+        log(f"unfused plug point (8192 reads): fp16 C {uf['fp16_C']['ms']} ms "
+            f"(gemm {uf['fp16_C']['gemm_ms']}, row top-k {uf['fp16_C']['rowtopk_ms']}), "
+            f"fp32 C {uf['fp32_C']['ms']} ms; fused {uf['fused_ms']} ms; fp32-C == fused: "
+            f"{uf['fp32_C'].get('bit_identical_to_fused')}")
+        del Bt, q8, engq
+        torch.cuda.empty_cache()
 
     # ---- FASTA ingestion: the batch as FASTA text, parsed and packed on the GPU
     if not args.no_fasta:

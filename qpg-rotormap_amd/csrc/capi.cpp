@@ -443,8 +443,17 @@ int rm_rowtopk_merge(rm_ctx *ctx, float *D_val, int32_t *D_loc, int64_t ldd, con
     RM_CHECK(k >= 1 && k <= 64, RM_ERR_ARG, "k must be 1..64");
     RM_CHECK(wi >= 1, RM_ERR_ARG, "chunk must be non-empty");
     RM_CHECK(ldd >= M && ldc >= M, RM_ERR_SHAPE, "leading dimensions < M");
-    return launch_rowtopk(D_val, D_loc, ldd, C, c_dtype, ldc, M, wi, offset, k,
-                          pick_stream(ctx, stream));
+    RM_CHECK(offset >= 0 && offset + wi <= ((int64_t)1 << 31) - 1, RM_ERR_ARG,
+             "column ids offset + wi must fit int32");
+    if (M == 0) return RM_OK;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const int G = rowtopk_segments(M, wi);
+    void *pv, *pc;
+    int rc;
+    if ((rc = ws_get(ctx, "rowtopk_v", (size_t)M * G * k * 4, &pv))) return rc;
+    if ((rc = ws_get(ctx, "rowtopk_c", (size_t)M * G * k * 4, &pc))) return rc;
+    return launch_rowtopk(D_val, D_loc, ldd, C, c_dtype, ldc, M, wi, offset, k, G, (float *)pv,
+                          (int32_t *)pc, pick_stream(ctx, stream));
 }
 
 int rm_index_create(rm_ctx *ctx, const uint8_t *B, int64_t rdim, int64_t N, int flags,

@@ -81,7 +81,7 @@ using ShapeTopk256 = Shape<512, 256, true>;
 using ShapeTopk128 = Shape<512, 128, true>;
 using ShapeTopk64 = Shape<512, 64, true>;
 using ShapeList128 = Shape<256, 128, false>;
-using ShapeStore = Shape<512, 256, false>;
+using ShapeStore = Shape<512, 256, kRingOk>;
 
 template <class SH>
 struct StageRegs {
@@ -553,12 +553,22 @@ __device__ __forceinline__ void store_item(const GemmArgs &a, int64_t rt, int sp
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wi = wid % SH::WI,
               wr = wid / SH::WI;
     uint8_t *sb0 = smem, *sb1 = smem + SH::STAGE_BYTES;
+    // the ring base is smem plus a wave-uniform opaque zero (keeps the
+    // fragment reads' alias scopes, as in the top-k kernel)
+    int zero = 0;
+    asm volatile("" : "+s"(zero));
+    uint8_t *const ring = smem + zero;
     const int64_t r0 = rt * SH::BR;
     const int64_t t0 = (int64_t)split * a.tiles / a.S, t1 = (int64_t)(split + 1) * a.tiles / a.S;
     v16f acc[AI][2];
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t i0 = t * kBI;
-        tile_product<ELEM, SH>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
+        // the ring's K loop ends with every load landed and every fragment
+        // read done (a barrier), so the next tile may refill it at once
+        if constexpr (SH::RING && kRingOk)
+            tile_product_ring<ELEM, SH, false>(acc, a, i0, r0, ring, tid, wi, wr, lane);
+        else
+            tile_product<ELEM, SH>(acc, a, i0, r0, sb0, sb1, tid, wi, wr, lane);
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) {
             const int64_t q = r0 + wr * 64 + bj * 32 + (lane & 31);
@@ -575,7 +585,7 @@ __device__ __forceinline__ void store_item(const GemmArgs &a, int64_t rt, int sp
                     }
                 }
         }
-        __syncthreads();  // the next tile's prologue refills the stage buffers
+        if constexpr (!(SH::RING && kRingOk)) __syncthreads();  // the next prologue refills the stages
     }
 }
 
@@ -1277,32 +1287,53 @@ int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int6
 }
 
 // ---------------------------------------------------------------------------
-// Drop-in running top-k over a C chunk (column-major, one thread per row):
-// the running list (earlier columns) merged with the chunk under the total
-// order.  Lists live in LDS, 64 rows per block.
+// Drop-in running top-k over a C chunk (column-major, rows = queries), in two
+// passes.  Pass 1: the chunk's columns are cut into G segments; a wave owns
+// 64 rows of one segment (lane = row: one coalesced 64-row load per column,
+// eight columns in flight) and keeps each row's top-k of its segment in LDS,
+// admitting only values that beat the k-th entry of the running list (when
+// full) and of the segment's own list.  Pass 2: one wave per row merges the G
+// segment lists and the running list (earlier columns) by a k-step
+// tournament under the total order (value desc, column asc), so the result
+// equals one sequential scan.  Lists of the segments go through a workspace
+// of M x G x k (value, column) pairs.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void rowtopk_kernel(float *dv, int32_t *dl, int64_t ldd,
-                                                     const void *C, int cdt, int64_t ldc, int64_t M,
-                                                     int64_t wi, int64_t offset, int k) {
+template <int CDT>
+__device__ __forceinline__ float load_c(const void *C, int64_t o) {
+    if constexpr (CDT == RM_F32) return ((const float *)C)[o];
+    else if constexpr (CDT == RM_F16) return f16_bits_to_f32(((const uint16_t *)C)[o]);
+    else return e4m3_to_f32(((const uint8_t *)C)[o]);
+}
+
+template <int CDT>
+__global__ __launch_bounds__(64) void rowtopk_seg_kernel(const void *C, int64_t ldc,
+                                                         int64_t M, int64_t wi, int64_t offset,
+                                                         int k, int G, const float *dv,
+                                                         const int32_t *dl, int64_t ldd,
+                                                         float *pv, int32_t *pc) {
+    constexpr int U = 8;  // columns per step: their loads are issued together
     extern __shared__ float lsh[];
     const int t = threadIdx.x;
     const int64_t row = (int64_t)blockIdx.x * 64 + t;
-    if (row >= M) return;
+    const int g = blockIdx.y;
+    const int64_t j0 = (int64_t)g * wi / G, j1 = (int64_t)(g + 1) * wi / G;
     float *L = lsh + t * (2 * k);
     int32_t *Lc = (int32_t *)(L + k);
-    for (int j = 0; j < k; ++j) {
-        const int32_t l = dl[row + j * ldd];
-        L[j] = dv[row + j * ldd];
-        Lc[j] = l > 0 ? l - 1 : INT_MAX;
+    for (int e = 0; e < k; ++e) { L[e] = -INFINITY; Lc[e] = INT_MAX; }
+    if (row >= M) return;
+    // Admission threshold: the running list's k-th entry when the list is
+    // full (a column that does not beat it cannot reach the merged top-k),
+    // else the segment list's own k-th entry once that fills.
+    float tv = -INFINITY;
+    int32_t tc = INT_MAX;
+    {
+        const int32_t l = dl[row + (int64_t)(k - 1) * ldd];
+        if (l > 0) {
+            tv = dv[row + (int64_t)(k - 1) * ldd];
+            tc = l - 1;
+        }
     }
-    for (int64_t j = 0; j < wi; ++j) {
-        const int64_t o = row + j * ldc;
-        float v;
-        if (cdt == RM_F32) v = ((const float *)C)[o];
-        else if (cdt == RM_F16) v = f16_bits_to_f32(((const uint16_t *)C)[o]);
-        else v = e4m3_to_f32(((const uint8_t *)C)[o]);
-        const int32_t c = (int32_t)(offset + j);
-        if (!beats(v, c, L[k - 1], Lc[k - 1])) continue;
+    auto insert = [&](float v, int32_t c) {
         int pos = k - 1;
         while (pos > 0 && beats(v, c, L[pos - 1], Lc[pos - 1])) {
             L[pos] = L[pos - 1];
@@ -1311,21 +1342,112 @@ __global__ __launch_bounds__(64) void rowtopk_kernel(float *dv, int32_t *dl, int
         }
         L[pos] = v;
         Lc[pos] = c;
+        if (beats(L[k - 1], Lc[k - 1], tv, tc)) {
+            tv = L[k - 1];
+            tc = Lc[k - 1];
+        }
+    };
+    const int64_t base = row + j0 * ldc;
+    int64_t j = j0;
+    for (; j + U <= j1; j += U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = load_c<CDT>(C, base + (j - j0 + u) * ldc);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t c = (int32_t)(offset + j + u);
+            if (beats(v[u], c, tv, tc)) insert(v[u], c);
+        }
     }
-    for (int j = 0; j < k; ++j) {
-        dv[row + j * ldd] = L[j];
-        dl[row + j * ldd] = Lc[j] == INT_MAX ? 0 : Lc[j] + 1;
+    for (; j < j1; ++j) {
+        const float v = load_c<CDT>(C, base + (j - j0) * ldc);
+        const int32_t c = (int32_t)(offset + j);
+        if (beats(v, c, tv, tc)) insert(v, c);
+    }
+    const int64_t o = ((int64_t)g * M + row) * k;
+    for (int e = 0; e < k; ++e) { pv[o + e] = L[e]; pc[o + e] = Lc[e]; }
+}
+
+// One wave per row: the running list (D_val / D_loc, 1-based ids, 0 =
+// empty) and the G segment lists, each sorted; lane u < G + 1 holds the head
+// of list u; k steps of a wave-wide argmax under the total order.
+__global__ __launch_bounds__(64) void rowtopk_final_kernel(float *dv, int32_t *dl, int64_t ldd,
+                                                           int64_t M, int k, int G,
+                                                           const float *pv, const int32_t *pc) {
+    __shared__ float ov[kMaxKP];
+    __shared__ int32_t oc[kMaxKP];
+    const int64_t row = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (row >= M) return;
+    int hp = 0;  // this lane's list position
+    auto head = [&](float &v, int32_t &c) {
+        v = -INFINITY;
+        c = INT_MAX;
+        if (hp >= k) return;
+        if (lane == 0) {
+            const int32_t l = dl[row + (int64_t)hp * ldd];
+            v = dv[row + (int64_t)hp * ldd];
+            c = l > 0 ? l - 1 : INT_MAX;
+        } else if (lane <= G) {
+            const int64_t o = ((int64_t)(lane - 1) * M + row) * k + hp;
+            v = pv[o];
+            c = pc[o];
+        }
+    };
+    float hv;
+    int32_t hc;
+    head(hv, hc);
+    for (int e = 0; e < k; ++e) {
+        float wv = hv;
+        int32_t wc = hc;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const float xv = __shfl_xor(wv, off, 64);
+            const int32_t xc = __shfl_xor(wc, off, 64);
+            if (beats(xv, xc, wv, wc)) { wv = xv; wc = xc; }
+        }
+        if (lane == 0) { ov[e] = wv; oc[e] = wc; }
+        if (wc != INT_MAX && hc == wc && hv == wv) {  // columns are unique: one lane advances
+            ++hp;
+            head(hv, hc);
+        }
+    }
+    __syncthreads();
+    for (int e = lane; e < k; e += 64) {
+        dv[row + (int64_t)e * ldd] = ov[e];
+        dl[row + (int64_t)e * ldd] = oc[e] == INT_MAX ? 0 : oc[e] + 1;
     }
 }
 
+// segments per chunk: about 8192 waves in pass 1 (enough loads in flight to
+// stream C from HBM), at least 256 columns each
+int rowtopk_segments(int64_t M, int64_t wi) {
+    const int64_t rb = cdiv(M, 64);
+    int64_t G = cdiv(8192, rb);
+    G = std::min<int64_t>(G, std::max<int64_t>(1, wi / 256));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(G, 63));
+}
+
 int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dtype, int64_t ldc,
-                   int64_t M, int64_t wi, int64_t offset, int k, hipStream_t st) {
+                   int64_t M, int64_t wi, int64_t offset, int k, int G, float *pv, int32_t *pc,
+                   hipStream_t st) {
     if (M == 0) return RM_OK;
-    RM_CHECK(cdiv(M, 64) * 64 <= kMaxWorkItems, RM_ERR_SHAPE, "%lld rows exceed one launch",
-             (long long)M);
-    size_t lds = (size_t)64 * 2 * k * 4;
-    rowtopk_kernel<<<dim3((unsigned)cdiv(M, 64)), dim3(64), lds, st>>>(dv, dl, ldd, C, c_dtype, ldc,
-                                                                       M, wi, offset, k);
+    RM_CHECK(G >= 1 && G <= 63, RM_ERR_ARG, "segments must be 1..63 (got %d)", G);
+    RM_CHECK(cdiv(M, 64) <= 0x7fffffffLL && M <= 0x7fffffffLL && M * 64 <= kMaxWorkItems,
+             RM_ERR_SHAPE, "%lld rows exceed one launch", (long long)M);
+    const size_t lds = (size_t)64 * 2 * k * 4;
+    const dim3 grid((unsigned)cdiv(M, 64), (unsigned)G);
+    if (c_dtype == RM_F32)
+        rowtopk_seg_kernel<RM_F32><<<grid, dim3(64), lds, st>>>(C, ldc, M, wi, offset, k, G, dv, dl,
+                                                               ldd, pv, pc);
+    else if (c_dtype == RM_F16)
+        rowtopk_seg_kernel<RM_F16><<<grid, dim3(64), lds, st>>>(C, ldc, M, wi, offset, k, G, dv, dl,
+                                                               ldd, pv, pc);
+    else
+        rowtopk_seg_kernel<RM_E4M3><<<grid, dim3(64), lds, st>>>(C, ldc, M, wi, offset, k, G, dv,
+                                                                dl, ldd, pv, pc);
+    RM_HIP(hipGetLastError());
+    rowtopk_final_kernel<<<dim3((unsigned)M), dim3(64), 0, st>>>(dv, dl, ldd, M, k, G, pv, pc);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }

@@ -120,7 +120,8 @@ inline int topk_cand_slots(int kp, int bq) {
 inline size_t topk_lds_bytes(int kp, int bq) {
     return uses_ring(kp) ? ring_lds_bytes(bq) : topk_lds_bytes_for(ring_list(kp) ? 256 : 128, kp);
 }
-inline size_t store_lds_bytes() { return 2 * (size_t)(kBI + 256) * kRow; }
+// the C-writing kernel: the 256-query ring (or two register-staged stages)
+inline size_t store_lds_bytes() { return kRingOk ? ring_bytes(256) : 2 * (size_t)(kBI + 256) * kRow; }
 
 // Tile plan of one fused launch: query rows per tile, column splits and the
 // query tiles grouped per XCD (0 = all).
@@ -172,8 +173,12 @@ int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st);
 int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int64_t N,
                  int ktop, int mode, const float *q32, int64_t ldq32, const uint8_t *db,
                  int64_t ldb, int K, float *vals, int32_t *locs, int64_t ldo, hipStream_t st);
+// the running row top-k of a C chunk in two passes: G column segments per
+// chunk (rowtopk_segments), their lists in pv / pc (M x G x k each)
+int rowtopk_segments(int64_t M, int64_t wi);
 int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dtype, int64_t ldc,
-                   int64_t M, int64_t wi, int64_t offset, int k, hipStream_t st);
+                   int64_t M, int64_t wi, int64_t offset, int k, int G, float *pv, int32_t *pc,
+                   hipStream_t st);
 int launch_transpose_u8(uint8_t *dst, const uint8_t *src, int64_t rows, int64_t cols,
                         int64_t s_row, int64_t s_col, int esize, hipStream_t st);
 

@@ -283,6 +283,49 @@ def test_rowtopk_merge_dropin(torch_gpu):
         np.testing.assert_array_equal(Dv.cpu().numpy(), ev)
 
 
+@pytest.mark.parametrize("M,k", [(100, 64), (700, 20), (64, 1)])
+def test_rowtopk_merge_segments_and_ties(torch_gpu, M, k):
+    """Chunks wide enough that the row top-k splits each chunk into many
+    column segments, merged afterwards with the running list: coarse e4m3
+    operands give many equal scores, so the column-ascending tie order across
+    segments and chunks is exercised; a ragged last chunk included."""
+    torch = torch_gpu
+    N, K, w = 11_000, 64, 4096
+    rng = np.random.default_rng(M + k)
+    qc = O.f32_to_e4m3([-0.5, 0.0, 0.5, 1.0])
+    bc = O.f32_to_e4m3([0.0, 0.25, 0.5])
+    Q = qc[rng.integers(0, 4, (M, K))]
+    B = bc[rng.integers(0, 3, (N, K))]
+    dq, db = torch.from_numpy(Q).cuda(), torch.from_numpy(B).cuda()
+    for cdt in (torch.float32, torch.float16):
+        Dv = torch.full((k, M), float("-inf"), device="cuda").t()
+        Dl = torch.zeros(k, M, dtype=torch.int32, device="cuda").t()
+        Call = np.zeros((M, N), dtype=np.float32)
+        buf = torch.zeros(w, M, dtype=cdt, device="cuda").t()
+        for lo in range(0, N, w):
+            wi = min(N, lo + w) - lo
+            R.fp8_gemm_hip(buf[:, :wi], dq, db, wi, b_col0=lo)
+            R.launch_seg_rowtopk_merge(Dv, Dl, buf[:, :wi], lo, k)
+            torch.cuda.synchronize()
+            Call[:, lo:lo + wi] = buf[:, :wi].float().cpu().numpy()
+        assert len(np.unique(Call[0])) < N // 10  # heavy ties
+        ev, el = O.topk_rows(Call, k)
+        np.testing.assert_array_equal(Dl.cpu().numpy(), el)
+        np.testing.assert_array_equal(Dv.cpu().numpy(), ev)
+    # e4m3 C chunks (random finite codes, both signs)
+    codes = rng.integers(0, 0x7f, (M, 2 * w + 5)).astype(np.uint8)
+    codes |= (rng.random(codes.shape) < 0.5).astype(np.uint8) << 7
+    Dv = torch.full((k, M), float("-inf"), device="cuda").t()
+    Dl = torch.zeros(k, M, dtype=torch.int32, device="cuda").t()
+    for lo in range(0, codes.shape[1], w):
+        ch = codes[:, lo:lo + w]
+        R.launch_seg_rowtopk_merge(Dv, Dl, torch.from_numpy(ch.T.copy()).cuda().t(), lo, k)
+    torch.cuda.synchronize()
+    ev, el = O.topk_rows(O.e4m3_table()[codes], k)
+    np.testing.assert_array_equal(Dl.cpu().numpy(), el)
+    np.testing.assert_array_equal(Dv.cpu().numpy(), ev)
+
+
 def test_persistent_grid_bitexact(torch_gpu):
     """One workgroup per CU walking the (row tile, split) items and one
     workgroup per item give identical top-k, also when each workgroup runs
