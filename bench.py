@@ -292,8 +292,14 @@ def main():
     el16, kms16, nl16, flop16, (v16, l16) = run(eng16, words, M, args.fp16_steps, 1,
                                                 "fp16 (verbatim fp16 index)")
     el16w, _, _, _, (v16w, l16w) = run(eng16w, words, M, 1, 1, "fp16 (widened e4m3 index)")
+    # FP8 candidates from the e4m3 copy of the fp16 index, FP32 rescoring
+    # against its verbatim fp16 columns (the FP16 mode ignores the setting)
+    B16v.set_rescore("fp16")
+    eng8r16 = R.TopKEngine(B16v, k=20, kprime=32, mode="fp8_rescore", rows_cap=M)
+    el8r16, _, _, _, (v8r16, l8r16) = run(eng8r16, words, M, 1, 1,
+                                          "fp8 + rescore on the fp16 columns")
     out_modes = {"fp8_rescore": (v8, l8), "fp8": (v8p, l8p), "fp16": (v16, l16),
-                 "fp16_widened": (v16w, l16w)}
+                 "fp16_widened": (v16w, l16w), "fp8_rescore_f16cols": (v8r16, l8r16)}
     out_modes = {k: (v.cpu().numpy(), l.cpu().numpy()) for k, (v, l) in out_modes.items()}
 
     def recall(locs, recs, starts, ktop=20):
@@ -336,7 +342,8 @@ def main():
         "recall_timed_batch_fp8_rescore": rnd(rec_all, 5),
         "reads_per_s": {"fp8_rescore": round(value, 1), "fp8": round(M / el8p, 1),
                         "fp16": round(M * args.fp16_steps / el16, 1),
-                        "fp16_widened": round(M / el16w, 1)},
+                        "fp16_widened": round(M / el16w, 1),
+                        "fp8_rescore_f16cols": round(M / el8r16, 1)},
         "index_build_s": rnd(t_index, 2),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP8_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP8_TFLOPS, 4),
@@ -361,8 +368,8 @@ def main():
         acc["top1_best_window"] = {}
         bcol, tcol = best_window_cols(ref, acc_reads.recs, acc_reads.starts, K_FRAG, KSTEP, 8192)
         for name, eng in (("fp8_rescore", eng8), ("fp8", eng8p), ("fp16", eng16),
-                          ("fp16_widened", eng16w)):
-            eng_cap = R.TopKEngine(eng.index, k=20, kprime=32, mode=name.split("_w")[0], rows_cap=na)
+                          ("fp16_widened", eng16w), ("fp8_rescore_f16cols", eng8r16)):
+            eng_cap = R.TopKEngine(eng.index, k=20, kprime=32, mode=eng.mode_name, rows_cap=na)
             v, l = eng_cap.map_words(re, aw, na)
             torch.cuda.synchronize()
             l = l.cpu().numpy()
@@ -380,6 +387,10 @@ def main():
         acc["fp8_rescore_minus_fp16_top1_reads"] = {
             str(e): int(round((acc["top1"]["fp8_rescore"][str(e)] - acc["top1"]["fp16"][str(e)]) * per))
             for e in ACC_ERRS}
+        for t in ("top20", "top1"):
+            acc[f"fp8_rescore_f16cols_minus_fp16_{t}_reads"] = {
+                str(e): int(round((acc[t]["fp8_rescore_f16cols"][str(e)] - acc[t]["fp16"][str(e)]) * per))
+                for e in ACC_ERRS}
         result["recall_by_error"] = acc
         log(f"accuracy top-20: {acc['top20']}")
         log(f"accuracy top-1: {acc['top1']}")
@@ -659,9 +670,10 @@ def main():
         q_ops = {"fp8": O.e4m3_table()[e8.cpu().numpy()].astype(np.float64),
                  "fp8_rescore": e32.cpu().numpy().astype(np.float64),
                  "fp16": e16.cpu().numpy().astype(np.float64),
-                 "fp16_widened": e16.cpu().numpy().astype(np.float64)}
+                 "fp16_widened": e16.cpu().numpy().astype(np.float64),
+                 "fp8_rescore_f16cols": e32.cpu().numpy().astype(np.float64)}
         b_ops = {"fp8": idx.embeds8, "fp8_rescore": idx.embeds8, "fp16": idx.embeds16,
-                 "fp16_widened": idx.embeds8}
+                 "fp16_widened": idx.embeds8, "fp8_rescore_f16cols": idx.embeds16}
         col_cache = {}
 
         def ideal_col(c1):
@@ -686,7 +698,8 @@ def main():
             serr["vs_ideal_fp64"][name] = {"max": float(d_ideal.max()), "mean": float(d_ideal.mean())}
         serr["note"] = ("same_operands: |GPU score - FP64 dot of the GPU's own query and stored "
                         "column| (fp8: e4m3 x e4m3, fp8_rescore: fp32 x e4m3, fp16: fp16 x verbatim "
-                        "fp16, fp16_widened: fp16 x e4m3-as-fp16) = the arithmetic error; "
+                        "fp16, fp16_widened: fp16 x e4m3-as-fp16, fp8_rescore_f16cols: fp32 x "
+                        "verbatim fp16) = the arithmetic error; "
                         "vs_ideal_fp64: against the FP64 golden encoder's query and column "
                         "(no quantisation) = arithmetic + quantisation error; over each read's 20 "
                         "returned columns")

@@ -145,6 +145,16 @@ int64_t rm_index_rdim(const rm_index *idx);
 /* Device pointers of the resident copies (fp16 NULL unless RM_INDEX_FP16). */
 const uint8_t *rm_index_e4m3(const rm_index *idx);
 const void *rm_index_f16(const rm_index *idx);
+/* Columns the FP32 rescoring of RM_MODE_FP8_RESCORE reads: RM_E4M3 (the
+ * default: the e4m3 database the candidates were ranked on) or RM_F16 (the
+ * index's fp16 columns; for an index from rm_index_create_f16 these are the
+ * verbatim fp16 columns, so the final scores and ranking carry the fp16
+ * index's precision while the k' candidates still come from the FP8 GEMM).
+ * RM_ERR_STATE if the index holds no fp16 columns.  Same fixed FP32 order
+ * as the e4m3 rescoring.  No counterpart in the reference, which has no
+ * rescoring stage.  rm_index_rescore returns the current setting. */
+int rm_index_set_rescore(rm_index *idx, int dtype);
+int rm_index_rescore(const rm_index *idx);
 
 /* ---- fused GEMM + top-k over the resident index ---------------------------
  * Replaces batch_gemm_topk! (RotorMap/search/engine_fp8.jl:399-437; fp16

@@ -47,6 +47,8 @@ def lib():
             "rmo_topk_rows_f32": (None, [vp, i64, i64, i64, i32, vp, vp]),
             "rmo_rescore_dot": (C.c_float, [vp, vp, i32]),
             "rmo_rescore_rank": (None, [vp, vp, i32, vp, i32, i32, vp, vp]),
+            "rmo_rescore_dot_t": (C.c_float, [vp, vp, i32, i32]),
+            "rmo_rescore_rank_t": (None, [vp, vp, i32, i32, vp, i32, i32, vp, vp]),
             "rmo_fasta_reads": (i64, [vp, i64, i32, vp, i64, vp, vp]),
             "rmo_cpu_map": (None, [vp, i64, i32, i32, i32, i32, vp, i64, i32, i32, vp, vp, i32]),
             "rmo_golden_query": (None, [vp, i32, i32, i32, i32, i32, vp]),
@@ -184,20 +186,25 @@ def topk_rows(Cm, k):
 
 
 def rescore_rank(q32, B, cand, ktop):
-    """Fixed-order FP32 rescoring of 1-based candidate columns -> (vals, locs)."""
+    """Fixed-order FP32 rescoring of 1-based candidate columns -> (vals, locs).
+    B: (N, K) uint8 e4m3 columns, or float16 columns (an fp16 index's verbatim
+    columns)."""
     q = np.ascontiguousarray(q32, dtype=np.float32)
-    B = np.ascontiguousarray(B, dtype=np.uint8)
+    elem = 1 if B.dtype == np.float16 else 0
+    B = np.ascontiguousarray(B, dtype=np.float16 if elem else np.uint8)
     c = np.ascontiguousarray(cand, dtype=np.int32)
     v = np.empty(ktop, dtype=np.float32)
     l = np.empty(ktop, dtype=np.int32)
-    lib().rmo_rescore_rank(_p(q), _p(B), B.shape[1], _p(c), len(c), ktop, _p(v), _p(l))
+    lib().rmo_rescore_rank_t(_p(q), _p(B), elem, B.shape[1], _p(c), len(c), ktop, _p(v), _p(l))
     return v, l
 
 
 def rescore_dot(q32, col):
+    """The rescoring dot of one column: uint8 e4m3 or float16 elements."""
     q = np.ascontiguousarray(q32, dtype=np.float32)
-    b = np.ascontiguousarray(col, dtype=np.uint8)
-    return lib().rmo_rescore_dot(_p(q), _p(b), len(q))
+    elem = 1 if col.dtype == np.float16 else 0
+    b = np.ascontiguousarray(col, dtype=np.float16 if elem else np.uint8)
+    return lib().rmo_rescore_dot_t(_p(q), _p(b), elem, len(q))
 
 
 def fasta_reads(raw, k):

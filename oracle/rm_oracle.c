@@ -462,14 +462,18 @@ void rmo_topk_rows_f32(const float *C, int64_t M, int64_t N, int64_t ldc, int k,
 /* The rescoring dot in its fixed FP32 order: 64 lanes, lane l owns elements
  * [l*C, (l+1)*C) (C = ceil(K/64)), each a sequential sum of q*deq(b) products
  * (multiply, then add: two roundings), folded by an xor butterfly (off = 32..1,
- * s[l] = s[l] + s[l^off]).  Returns the value every lane ends with. */
-float rmo_rescore_dot(const float *q, const uint8_t *b, int K) {
+ * s[l] = s[l] + s[l^off]).  Returns the value every lane ends with.  The
+ * column b holds K e4m3 bytes (elem 0) or K binary16 values (elem 1: the
+ * verbatim fp16 columns of an fp16 index). */
+float rmo_rescore_dot_t(const float *q, const void *b, int elem, int K) {
     float s[64], t[64];
     int C = (K + 63) / 64;
     for (int l = 0; l < 64; ++l) {
         float acc = 0.0f;
         for (int j = l * C; j < (l + 1) * C && j < K; ++j) {
-            float p = q[j] * rmo_e4m3_to_f32(b[j]);
+            float d = elem == 1 ? rmo_f16_to_f32(((const uint16_t *)b)[j])
+                                : rmo_e4m3_to_f32(((const uint8_t *)b)[j]);
+            float p = q[j] * d;
             acc = acc + p;
         }
         s[l] = acc;
@@ -481,17 +485,23 @@ float rmo_rescore_dot(const float *q, const uint8_t *b, int K) {
     return s[0];
 }
 
-/* Rescore a candidate list: scores[j] = rmo_rescore_dot(q, B[col_j - 1]),
- * then order by (score desc, col asc) and keep the first ktop.  Empty
- * candidates (col 0) stay empty. */
-void rmo_rescore_rank(const float *q, const uint8_t *B, int K, const int32_t *cand, int nc,
-                      int ktop, float *vals, int32_t *locs) {
+float rmo_rescore_dot(const float *q, const uint8_t *b, int K) {
+    return rmo_rescore_dot_t(q, b, 0, K);
+}
+
+/* Rescore a candidate list: scores[j] = rmo_rescore_dot_t(q, column col_j - 1
+ * of B (N x K elements of type elem, column-contiguous)), then order by
+ * (score desc, col asc) and keep the first ktop.  Empty candidates (col 0)
+ * stay empty. */
+void rmo_rescore_rank_t(const float *q, const void *B, int elem, int K, const int32_t *cand,
+                        int nc, int ktop, float *vals, int32_t *locs) {
     float sc[1024];
     int32_t cc[1024];
+    const size_t esz = elem == 1 ? 2 : 1;
     int n = 0;
     for (int j = 0; j < nc && n < 1024; ++j) {
         if (cand[j] <= 0) continue;
-        sc[n] = rmo_rescore_dot(q, B + (int64_t)(cand[j] - 1) * K, K);
+        sc[n] = rmo_rescore_dot_t(q, (const uint8_t *)B + (size_t)(cand[j] - 1) * K * esz, elem, K);
         cc[n] = cand[j];
         ++n;
     }
@@ -501,6 +511,11 @@ void rmo_rescore_rank(const float *q, const uint8_t *B, int K, const int32_t *ca
         for (int j = 0; j < n; ++j) rank += rmo_beats(sc[j], cc[j], sc[i], cc[i]);
         if (rank < ktop) { vals[rank] = sc[i]; locs[rank] = cc[i]; }
     }
+}
+
+void rmo_rescore_rank(const float *q, const uint8_t *B, int K, const int32_t *cand, int nc,
+                      int ktop, float *vals, int32_t *locs) {
+    rmo_rescore_rank_t(q, B, 0, K, cand, nc, ktop, vals, locs);
 }
 
 /* ------------------------------------------------------------------------ */

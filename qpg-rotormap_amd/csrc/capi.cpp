@@ -211,8 +211,10 @@ int topk_core_bufs(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_
         ctx->prof_ev.push_back({e0, e1});
         ctx->prof_flop += 2.0 * (double)M * (double)K * (double)idx->N;
     }
+    const bool r16 = idx->rescore == RM_F16 && idx->b16;
     return launch_merge((const float *)pv, (const int32_t *)pc, S, KP, M, idx->N, ktop, mode, Q32,
-                        ldq32, idx->b8, K, (int)K, vals, locs, ldo, st);
+                        ldq32, r16 ? (const void *)idx->b16 : (const void *)idx->b8,
+                        r16 ? RM_F16 : RM_E4M3, K, (int)K, vals, locs, ldo, st);
 }
 
 int check_encoder(int k, int s, int m, int c, int normalize) {
@@ -512,6 +514,7 @@ int rm_index_create_f16(rm_ctx *ctx, const uint16_t *B16, int64_t rdim, int64_t 
     ix->ctx = ctx;
     ix->rdim = rdim;
     ix->N = N;
+    ix->verbatim16 = true;
     const size_t n = (size_t)rdim * N;
     if (hipMalloc(&ix->b16, n * 2) != hipSuccess || hipMalloc(&ix->b8, n) != hipSuccess) {
         if (ix->b16) hipFree(ix->b16);
@@ -550,6 +553,18 @@ int64_t rm_index_cols(const rm_index *idx) { return idx ? idx->N : 0; }
 int64_t rm_index_rdim(const rm_index *idx) { return idx ? idx->rdim : 0; }
 const uint8_t *rm_index_e4m3(const rm_index *idx) { return idx ? idx->b8 : nullptr; }
 const void *rm_index_f16(const rm_index *idx) { return idx ? idx->b16 : nullptr; }
+
+int rm_index_set_rescore(rm_index *idx, int dtype) {
+    RM_CHECK(idx, RM_ERR_ARG, "NULL index");
+    RM_CHECK(dtype == RM_E4M3 || dtype == RM_F16, RM_ERR_ARG,
+             "rescoring columns must be RM_E4M3 or RM_F16 (got %d)", dtype);
+    RM_CHECK(dtype == RM_E4M3 || idx->b16, RM_ERR_STATE,
+             "RM_F16 rescoring needs fp16 columns (rm_index_create_f16 or RM_INDEX_FP16)");
+    std::lock_guard<std::recursive_mutex> lk(idx->ctx->mu);
+    idx->rescore = dtype;
+    return RM_OK;
+}
+int rm_index_rescore(const rm_index *idx) { return idx ? idx->rescore : -1; }
 
 }  // extern "C"
 

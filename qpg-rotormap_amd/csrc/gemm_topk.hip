@@ -1141,13 +1141,24 @@ int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st) {
 // (lane l owns K elements [l*C, (l+1)*C), sequential multiply-then-add, xor
 // butterfly) and re-ranks under (score desc, column asc).
 // ---------------------------------------------------------------------------
+// rescoring column element: CE = RM_E4M3 (the e4m3 database) or RM_F16 (the
+// verbatim fp16 columns of an fp16 index); ldb in elements
+template <int CE>
+__device__ __forceinline__ float col_elem(const void *db, int64_t o) {
+    if constexpr (CE == RM_F16) return f16_bits_to_f32(((const uint16_t *)db)[o]);
+    else return e4m3_to_f32(((const uint8_t *)db)[o]);
+}
+
+template <int CE>
 __global__ __launch_bounds__(64) void merge_kernel(const float *__restrict__ pv,
                                                    const int32_t *__restrict__ pc, int S, int KP,
                                                    int64_t M, int64_t N, int ktop, int mode,
                                                    const float *__restrict__ q32, int64_t ldq32,
-                                                   const uint8_t *__restrict__ db, int64_t ldb, int K,
+                                                   const void *__restrict__ db, int64_t ldb, int K,
                                                    float *__restrict__ vals,
                                                    int32_t *__restrict__ locs, int64_t ldo) {
+    constexpr int ESZ = CE == RM_F16 ? 2 : 1;  // bytes per column element
+    const uint8_t *__restrict__ dbb = (const uint8_t *)db;
     __shared__ float mv[kMaxKP];
     __shared__ int32_t mc[kMaxKP];
     __shared__ float sc[kMaxKP];
@@ -1210,19 +1221,33 @@ __global__ __launch_bounds__(64) void merge_kernel(const float *__restrict__ pv,
 #pragma unroll
                 for (int t0 = 0; t0 < 32; t0 += 8) {
                     if (t0 >= KP) break;
-                    uint4 cv[8];
+                    uint4 cv[8][ESZ];  // 16 elements of each of 8 candidates
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         const int32_t c = (t0 + u < KP) ? mc[t0 + u] : INT_MAX;
-                        cv[u] = c == INT_MAX ? make_uint4(0, 0, 0, 0)
-                                             : *(const uint4 *)(db + (int64_t)c * ldb + e_lo + e0);
+#pragma unroll
+                        for (int h = 0; h < ESZ; ++h)
+                            cv[u][h] = c == INT_MAX
+                                           ? make_uint4(0, 0, 0, 0)
+                                           : *(const uint4 *)(dbb + ((int64_t)c * ldb + e_lo + e0) * ESZ +
+                                                              16 * h);
                     }
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        const uint32_t w[4] = {cv[u].x, cv[u].y, cv[u].z, cv[u].w};
+                        uint32_t w[4 * ESZ];
+#pragma unroll
+                        for (int h = 0; h < ESZ; ++h) {
+                            w[4 * h] = cv[u][h].x; w[4 * h + 1] = cv[u][h].y;
+                            w[4 * h + 2] = cv[u][h].z; w[4 * h + 3] = cv[u][h].w;
+                        }
 #pragma unroll
                         for (int j = 0; j < 16; ++j) {
-                            const float p = qv[j] * e4m3_byte_to_f32(w[j >> 2], j & 3);
+                            float d;
+                            if constexpr (CE == RM_F16)
+                                d = f16_bits_to_f32((uint16_t)(w[j >> 1] >> (16 * (j & 1))));
+                            else
+                                d = e4m3_byte_to_f32(w[j >> 2], j & 3);
+                            const float p = qv[j] * d;
                             acc[t0 + u] = acc[t0 + u] + p;
                         }
                     }
@@ -1242,10 +1267,10 @@ __global__ __launch_bounds__(64) void merge_kernel(const float *__restrict__ pv,
             for (int t = 0; t < KP; ++t) {
                 const int32_t c = mc[t];
                 if (c == INT_MAX) { if (lane == 0) sc[t] = -INFINITY; continue; }
-                const uint8_t *col = db + (int64_t)c * ldb;
+                const int64_t col = (int64_t)c * ldb;
                 float acc = 0.0f;
                 for (int e = e_lo; e < e_lo + C && e < K; ++e) {
-                    const float p = qs[e] * e4m3_to_f32(col[e]);
+                    const float p = qs[e] * col_elem<CE>(db, col + e);
                     acc = acc + p;
                 }
 #pragma unroll
@@ -1274,14 +1299,20 @@ __global__ __launch_bounds__(64) void merge_kernel(const float *__restrict__ pv,
 }
 
 int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int64_t N,
-                 int ktop, int mode, const float *q32, int64_t ldq32, const uint8_t *db,
-                 int64_t ldb, int K, float *vals, int32_t *locs, int64_t ldo, hipStream_t st) {
+                 int ktop, int mode, const float *q32, int64_t ldq32, const void *db,
+                 int db_dtype, int64_t ldb, int K, float *vals, int32_t *locs, int64_t ldo,
+                 hipStream_t st) {
     if (M == 0) return RM_OK;
     RM_CHECK(S <= 256, RM_ERR_ARG, "at most 256 column splits (got %d)", S);
     RM_CHECK(M * 64 <= kMaxWorkItems, RM_ERR_SHAPE, "%lld rows exceed one launch", (long long)M);
+    RM_CHECK(db_dtype == RM_E4M3 || db_dtype == RM_F16, RM_ERR_ARG, "rescoring columns: e4m3 or fp16");
     size_t lds = (mode == RM_MODE_FP8_RESCORE) ? (size_t)K * 4 : 0;
-    merge_kernel<<<dim3((unsigned)M), dim3(64), lds, st>>>(part_v, part_c, S, KP, M, N, ktop, mode,
-                                                           q32, ldq32, db, ldb, K, vals, locs, ldo);
+    if (db_dtype == RM_F16 && mode == RM_MODE_FP8_RESCORE)
+        merge_kernel<RM_F16><<<dim3((unsigned)M), dim3(64), lds, st>>>(
+            part_v, part_c, S, KP, M, N, ktop, mode, q32, ldq32, db, ldb, K, vals, locs, ldo);
+    else
+        merge_kernel<RM_E4M3><<<dim3((unsigned)M), dim3(64), lds, st>>>(
+            part_v, part_c, S, KP, M, N, ktop, mode, q32, ldq32, db, ldb, K, vals, locs, ldo);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }

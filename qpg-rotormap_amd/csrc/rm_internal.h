@@ -170,9 +170,12 @@ struct GemmArgs {
 // launchers (gemm_topk.hip)
 int launch_gemm_topk(const GemmArgs &a, int elem, hipStream_t st);
 int launch_gemm_store(const GemmArgs &a, int elem, hipStream_t st);
+// db: the columns the FP32 rescoring reads, db_dtype RM_E4M3 or RM_F16, ldb
+// elements apart
 int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int64_t M, int64_t N,
-                 int ktop, int mode, const float *q32, int64_t ldq32, const uint8_t *db,
-                 int64_t ldb, int K, float *vals, int32_t *locs, int64_t ldo, hipStream_t st);
+                 int ktop, int mode, const float *q32, int64_t ldq32, const void *db,
+                 int db_dtype, int64_t ldb, int K, float *vals, int32_t *locs, int64_t ldo,
+                 hipStream_t st);
 // the running row top-k of a C chunk in two passes: G column segments per
 // chunk (rowtopk_segments), their lists in pv / pc (M x G x k each)
 int rowtopk_segments(int64_t M, int64_t wi);
@@ -237,8 +240,10 @@ struct rm_ctx {
 struct rm_index {
     rm_ctx *ctx = nullptr;
     uint8_t *b8 = nullptr;    // N x rdim e4m3 (column j contiguous)
-    uint16_t *b16 = nullptr;  // N x rdim fp16 widening (optional)
+    uint16_t *b16 = nullptr;  // N x rdim fp16: the widening, or the verbatim fp16 index
     int64_t rdim = 0, N = 0;
+    bool verbatim16 = false;   // b16 holds the fp16 index itself (rm_index_create_f16)
+    int rescore = RM_E4M3;     // columns the FP32 rescoring reads: RM_E4M3 (b8) or RM_F16 (b16)
 };
 
 namespace rm {

@@ -57,6 +57,8 @@ SIGNATURES = {
     "rm_index_rdim": (i64, [vp]),
     "rm_index_e4m3": (vp, [vp]),
     "rm_index_f16": (vp, [vp]),
+    "rm_index_set_rescore": (i32, [vp, i32]),
+    "rm_index_rescore": (i32, [vp]),
     "rm_topk_batch": (i32, [vp, vp, vp, i64, vp, i64, i64, i32, i32, i32, vp, vp, i64, vp]),
     "rm_map_batch": (i32, [vp, vp, vp, i64, i32, i32, i32, i32, i32, i32, i32, vp, vp, i64, vp, vp]),
     "rm_map_batch_noalloc": (i32, [vp, vp, vp, i64, i32, i32, i32, i32, i32, i32, i32, vp, vp, i64,

@@ -54,6 +54,17 @@ class ResidentIndex:
         self.fp16 = bool(fp16 or verbatim)
         self.verbatim_f16 = verbatim
 
+    def set_rescore(self, dtype="e4m3"):
+        """Columns the FP32 rescoring (mode fp8_rescore) reads: "e4m3" (the
+        database the candidates were ranked on) or "fp16" (the index's fp16
+        columns: for a float16-built index the verbatim ones)."""
+        code = {"e4m3": L.RM_E4M3, "fp16": L.RM_F16}[dtype]
+        L.check(L.lib().rm_index_set_rescore(self.handle, code))
+
+    @property
+    def rescore(self):
+        return {L.RM_E4M3: "e4m3", L.RM_F16: "fp16"}[L.lib().rm_index_rescore(self.handle)]
+
     def close(self):
         if self.handle:
             L.lib().rm_index_destroy(self.handle)
@@ -94,6 +105,10 @@ class TopKEngine:
         if not (1 <= k <= N):
             raise L.RotorMapArgumentError(L.RM_ERR_ARG, f"need 1 <= k <= N (got k = {k}, N = {N})")
         self.k, self.kprime, self.rows_cap, self.device = int(k), int(max(kprime, k)), int(rows_cap), device
+
+    @property
+    def mode_name(self):
+        return {v: k for k, v in MODES.items()}[self.mode]
 
     def _out(self, M):
         import torch

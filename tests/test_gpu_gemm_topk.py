@@ -460,3 +460,42 @@ def test_fp16_verbatim_index(torch_gpu):
     a, b = fused(torch, eng8, Q8), fused(torch, eng8h, Q8)
     np.testing.assert_array_equal(a[1], b[1])
     np.testing.assert_array_equal(a[0], b[0])
+
+
+def test_rescore_on_verbatim_fp16_columns(torch_gpu):
+    """FP8 + rescore on an fp16 index with rm_index_set_rescore(RM_F16): the
+    k' candidates come from the FP8 GEMM over the e4m3 copy, and the final
+    scores are the fixed-order FP32 dots against the verbatim fp16 columns
+    (the oracle's rescore_rank over float16 columns), bit for bit, for every
+    tile shape; switching back to e4m3 restores the e4m3 rescoring."""
+    torch = torch_gpu
+    M, N, K, k, kp = 160, 2500, 2048, 20, 32
+    rng = np.random.default_rng(77)
+    x = rng.standard_normal((N, K))
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    B16 = x.astype(np.float16)
+    idx = R.ResidentIndex(B16)
+    assert idx.rescore == "e4m3"
+    B8 = O.f32_to_e4m3(B16.astype(np.float32).ravel()).reshape(N, K)
+    Q8, Q32 = unit_rows_e4m3(M, K, 78)
+    _, cand = O.topk_rows(gemm_c(torch, Q8, B8), kp)
+    eng = R.TopKEngine(idx, k=k, kprime=kp, mode="fp8_rescore")
+    idx.set_rescore("fp16")
+    assert idx.rescore == "fp16"
+    for tile in TILES:
+        v, l = fused(torch, eng, Q8, Q32, tile=tile)
+        for r in range(M):
+            ev, el = O.rescore_rank(Q32[r], B16, cand[r], k)
+            np.testing.assert_array_equal(l[r], el, err_msg=f"tile={tile} row {r}")
+            np.testing.assert_array_equal(v[r], ev, err_msg=f"tile={tile} row {r}")
+    ref = Q32.astype(np.float64) @ B16.astype(np.float64).T
+    assert np.max(np.abs(ref[np.arange(M)[:, None], l - 1] - v)) < 1e-5
+    idx.set_rescore("e4m3")
+    v, l = fused(torch, eng, Q8, Q32)
+    for r in range(0, M, 7):
+        ev, el = O.rescore_rank(Q32[r], B8, cand[r], k)
+        np.testing.assert_array_equal(l[r], el)
+        np.testing.assert_array_equal(v[r], ev)
+    # an e4m3-only index has no fp16 columns to rescore against
+    with pytest.raises(L.RotorMapError):
+        R.ResidentIndex(B8).set_rescore("fp16")

@@ -321,6 +321,37 @@ def test_rescore_dot_fixed_order():
     got = O.rescore_dot(q, b)
     assert got == s[0]
     assert abs(got - float(np.dot(q.astype(np.float64), d.astype(np.float64)))) < 1e-4
+    # fp16 columns: the same order over binary16 elements; the exact fp16
+    # widening of an e4m3 column gives the e4m3 column's bits
+    assert O.rescore_dot(q, d.astype(np.float16)) == got
+    h = (rng.standard_normal(K) * 0.03).astype(np.float16)
+    s = np.zeros(64, np.float32)
+    for l in range(64):
+        acc = np.float32(0)
+        for e in range(l * C_, (l + 1) * C_):
+            acc = np.float32(acc + np.float32(q[e] * np.float32(h[e])))
+        s[l] = acc
+    for off in (32, 16, 8, 4, 2, 1):
+        s = np.array([np.float32(s[l] + s[l ^ off]) for l in range(64)], np.float32)
+    assert O.rescore_dot(q, h) == s[0]
+
+
+def test_rescore_rank_fp16_columns_vs_numpy():
+    """Ranking of rescored fp16 columns: (score desc, column asc) with
+    duplicate columns forcing ties and an empty candidate slot."""
+    rng = np.random.default_rng(17)
+    K, N = 128, 40
+    B = (rng.standard_normal((N, K)) * 0.05).astype(np.float16)
+    B[7] = B[3]  # equal scores: column 4 ranks before column 8
+    q = rng.standard_normal(K).astype(np.float32)
+    cand = np.array([8, 4, 0, 12, 1, 30, 22, 40], np.int32)
+    v, l = O.rescore_rank(q, B, cand, 5)
+    live = cand[cand > 0]
+    sc = np.array([O.rescore_dot(q, B[c - 1]) for c in live], np.float32)
+    order = np.lexsort((live, -sc))[:5]
+    np.testing.assert_array_equal(l, live[order])
+    np.testing.assert_array_equal(v, sc[order])
+    assert list(l).index(4) < list(l).index(8) if 8 in l and 4 in l else True
 
 
 # ---- the CPU golden mapping path and the modes' score models --------------
