@@ -1348,7 +1348,9 @@ __global__ __launch_bounds__(64) void rowtopk_seg_kernel(const void *C, int64_t 
     const int64_t row = (int64_t)blockIdx.x * 64 + t;
     const int g = blockIdx.y;
     const int64_t j0 = (int64_t)g * wi / G, j1 = (int64_t)(g + 1) * wi / G;
-    float *L = lsh + t * (2 * k);
+    // a lane's list: k values then k columns, 2k + 1 words apart (an odd
+    // stride puts the 64 lanes' entries e in distinct LDS banks)
+    float *L = lsh + t * (2 * k + 1);
     int32_t *Lc = (int32_t *)(L + k);
     for (int e = 0; e < k; ++e) { L[e] = -INFINITY; Lc[e] = INT_MAX; }
     if (row >= M) return;
@@ -1466,7 +1468,7 @@ int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dty
     RM_CHECK(G >= 1 && G <= 63, RM_ERR_ARG, "segments must be 1..63 (got %d)", G);
     RM_CHECK(cdiv(M, 64) <= 0x7fffffffLL && M <= 0x7fffffffLL && M * 64 <= kMaxWorkItems,
              RM_ERR_SHAPE, "%lld rows exceed one launch", (long long)M);
-    const size_t lds = (size_t)64 * 2 * k * 4;
+    const size_t lds = (size_t)64 * (2 * k + 1) * 4;
     const dim3 grid((unsigned)cdiv(M, 64), (unsigned)G);
     if (c_dtype == RM_F32)
         rowtopk_seg_kernel<RM_F32><<<grid, dim3(64), lds, st>>>(C, ldc, M, wi, offset, k, G, dv, dl,
