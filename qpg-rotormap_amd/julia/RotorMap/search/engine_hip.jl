@@ -25,13 +25,18 @@
 
 """
     HipTopKEngine(B; k = 20, kprime = 32, mode = :fp8_rescore, rows_cap = 2^13,
-                  fp16 = mode === :fp16, host = false, device = 0)
+                  fp16 = mode === :fp16, host = false, rescore = :e4m3, device = 0)
 
 The resident database for the fused path: `B` (rdim, N) e4m3 columns (a
 device array, or a host matrix with host = true), as index_database returns
 it (RotorMap/index/load.jl:156-161); fp16 = true also keeps the exact fp16
-widening (index_database_f16, :130-143).  mode :fp8 (top-k of the e4m3
-GEMM), :fp8_rescore (top-kprime, then FP32 rescoring to k) or :fp16.
+widening (index_database_f16, :130-143).  A Float16 `B` is the classic fp16
+index (index_to_f16, :96-116), kept verbatim for :fp16 and quantised to
+e4m3(fp32(fp16)) on the device for the FP8 modes (index_to_f8, :76-89).
+mode :fp8 (top-k of the e4m3 GEMM), :fp8_rescore (top-kprime, then FP32
+rescoring to k) or :fp16.  rescore = :fp16 (Float16 `B`, or fp16 = true)
+rescores the kprime candidates against the fp16 columns instead of the e4m3
+ones.
 """
 mutable struct HipTopKEngine
     index::Ptr{Cvoid}
@@ -46,23 +51,67 @@ end
 
 function HipTopKEngine(B; k::Integer = 20, kprime::Integer = 32, mode::Symbol = :fp8_rescore,
                        rows_cap::Integer = 2^13, fp16::Bool = mode === :fp16, host::Bool = false,
-                       device::Integer = 0)
+                       rescore::Symbol = :e4m3, device::Integer = 0)
     rdim, N = size(B)
     1 <= k <= N || throw(ArgumentError("need 1 <= k <= N (got k = $k, N = $N)"))
     haskey(HIP_MODES, mode) || throw(ArgumentError("mode must be :fp8, :fp8_rescore or :fp16"))
-    sizeof(eltype(B)) == 1 || throw(ArgumentError("B must hold 1-byte e4m3 elements"))
-    flags = (host ? HIP_INDEX_HOST : Cint(0)) | (fp16 ? HIP_INDEX_FP16 : Cint(0))
+    rescore in (:e4m3, :fp16) || throw(ArgumentError("rescore must be :e4m3 or :fp16"))
+    esz = sizeof(eltype(B))
+    esz in (1, 2) || throw(ArgumentError("B must hold e4m3 (1-byte) or Float16 elements"))
+    src = host ? Ptr{Cvoid}(pointer(B)) : hip_devptr(B)
     out = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve B hip_check(ccall((:rm_index_create, HIP_LIB), Cint,
-                                   (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Cint, Ref{Ptr{Cvoid}}),
-                                   hip_ctx(device), host ? Ptr{Cvoid}(pointer(B)) : hip_devptr(B),
-                                   rdim, N, flags, out))
+    if esz == 2  # the verbatim fp16 index (+ its e4m3 quantisation)
+        GC.@preserve B hip_check(ccall((:rm_index_create_f16, HIP_LIB), Cint,
+                                       (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Cint, Ref{Ptr{Cvoid}}),
+                                       hip_ctx(device), src, rdim, N,
+                                       host ? HIP_INDEX_HOST : Cint(0), out))
+    else
+        flags = (host ? HIP_INDEX_HOST : Cint(0)) | (fp16 ? HIP_INDEX_FP16 : Cint(0))
+        GC.@preserve B hip_check(ccall((:rm_index_create, HIP_LIB), Cint,
+                                       (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Cint, Ref{Ptr{Cvoid}}),
+                                       hip_ctx(device), src, rdim, N, flags, out))
+    end
     eng = HipTopKEngine(out[], Int(device), Int(k), Int(max(kprime, k)), mode, Int(rows_cap),
                         Int(rdim), Int(N))
     finalizer(e -> (e.index != C_NULL && ccall((:rm_index_destroy, HIP_LIB), Cint, (Ptr{Cvoid},),
                                                e.index); e.index = C_NULL), eng)
+    rescore === :fp16 && hip_check(ccall((:rm_index_set_rescore, HIP_LIB), Cint,
+                                         (Ptr{Cvoid}, Cint), eng.index, HIP_F16))
     return eng
 end
+
+"""
+    hip_autotune!(eng, bucket_rows) -> Vector{NamedTuple}
+
+Time the candidate tile plans (query rows per tile, column splits, XCD query
+grouping) for batches of each size in `bucket_rows` on the engine's index and
+keep the fastest per size bucket in the device's context; later launches of
+that bucket use it.  Results never depend on the plan.  Returns the chosen
+(tile, splits, qgroup, ms) per bucket.
+"""
+function hip_autotune!(eng::HipTopKEngine, bucket_rows::AbstractVector{<:Integer})
+    nb = length(bucket_rows)
+    rows = Int64.(bucket_rows)
+    tile, sp, qg = zeros(Int32, nb), zeros(Int32, nb), zeros(Int32, nb)
+    ms = zeros(Float32, nb)
+    hip_check(ccall((:rm_autotune, HIP_LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int64}, Cint, Cint, Cint, Cint,
+                     Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Float32}),
+                    hip_ctx(eng.device), eng.index, rows, nb, HIP_MODES[eng.mode], eng.k,
+                    eng.kprime, tile, sp, qg, ms))
+    return map(i -> (tile = Int(tile[i]), splits = Int(sp[i]), qgroup = Int(qg[i]), ms = ms[i]), 1:nb)
+end
+
+"""
+    hip_plan!(eng, M; tile = 0, splits = 0, qgroup = 0)
+
+Pin the tile plan of M's size bucket (tile 64 / 128 / 256 query rows or 0 for
+the default, splits 0 for the built-in rule); tile = splits = 0 clears it.
+"""
+hip_plan!(eng::HipTopKEngine, M::Integer; tile::Integer = 0, splits::Integer = 0,
+          qgroup::Integer = 0) =
+    hip_check(ccall((:rm_plan_set, HIP_LIB), Cint, (Ptr{Cvoid}, Int64, Cint, Cint, Cint, Cint),
+                    hip_ctx(eng.device), M, HIP_MODES[eng.mode], tile, splits, qgroup))
 
 """
     batch_gemm_topk_hip!(D_val, D_loc, eng, Q; Q32 = nothing, stream = C_NULL)
