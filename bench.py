@@ -37,7 +37,8 @@ Legs beside the headline step:
                     one persistent launch, after the autotuner
   serving           Poisson traces of 400 requests at 10 / 20 / 50 / 200 /
                     1000 req/s (reads per request log-uniform on [1, 100,000],
-                    err in [0, 0.10]) through the continuous-batching server:
+                    err in [0, 0.10]) through the continuous-batching server
+                    (16,384-row launches, 2 slots, SRPT):
                     p50 / p99 / p999, offered vs achieved reads/s, recall
   fasta             the batch as FASTA text, parsed and 2-bit packed on the GPU
   cpu_baseline      the oracle's golden path (FP64 encoder, FP64 dots, top-20)
@@ -185,6 +186,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=400, help="requests per rate")
     ap.add_argument("--serve-rows", type=int, default=16384, help="rows per serving launch")
     ap.add_argument("--serve-streams", type=int, default=2, help="serving launch slots")
+    ap.add_argument("--serve-lowload", default="0,0",
+                    help="launch_rows,backlog_rows of the low-load launch cap (0,0 = off)")
     ap.add_argument("--serve-policy", choices=["fifo", "srpt"], default="srpt",
                     help="order in which queued rows fill a launch")
     ap.add_argument("--uneven-batches", type=int, default=512,
@@ -619,13 +622,15 @@ def main():
     if not args.no_serve:
         pinned = torch.from_numpy(spool.words.view(np.int32)).pin_memory()
         rates = [float(x) for x in serve_rates.split(",")]
+        lowload = tuple(int(x) for x in args.serve_lowload.split(","))
         serving = {"mode": "fp8_rescore", "rows_per_launch": args.serve_rows,
                    "launch_slots": args.serve_streams, "policy": args.serve_policy,
+                   "lowload_launch_rows_backlog_rows": list(lowload),
                    "request_reads": "log-uniform on [1, 100000]", "err": "uniform [0, 0.10]",
                    "pool_reads": len(spool.recs), "rates": []}
         with R.Server(B8, re, k=20, kprime=32, mode="fp8_rescore",
                       max_rows=args.serve_rows, n_streams=args.serve_streams,
-                      policy=args.serve_policy) as srv:
+                      policy=args.serve_policy, lowload=lowload) as srv:
             for i, rate in enumerate(rates):
                 t, n = R.poisson_trace(args.serve_requests, rate, seed=100 + i)
                 res = R.run_trace(srv, pinned, t, n, seed=200 + i, keep_results=True)

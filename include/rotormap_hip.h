@@ -212,7 +212,7 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words,
  * tiles grouped per XCD qg) per (mode, batch-size bucket ceil(log2 M)); the
  * reference's fixed chunk width w and rows_cap (RotorMap/search/
  * engine_fp8.jl:309-337) are what it replaces.  Without a tuned entry a
- * batch takes the smallest tile that holds it (256 from 129 rows) and the
+ * batch takes 64 rows up to 64, 128 up to 256, else 256, and the
  * built-in split rule; lists longer than 32 (kp > 32) always run 128-row
  * tiles.  Results never depend on the plan.
  * rm_plan_get: the plan a batch of M rows gets; kp is the list length the
@@ -303,6 +303,14 @@ int rm_server_stats(rm_server *srv, int64_t *launches, int64_t *rows_done, int64
 #define RM_SCHED_FIFO 0
 #define RM_SCHED_SRPT 1
 int rm_server_set_policy(rm_server *srv, int policy);
+/* Low-load launch cap: while at most backlog_rows rows wait unlaunched, a
+ * launch takes at most launch_rows rows (0 = off, the default).  The fused
+ * kernel holds every CU while it runs, so a request that arrives behind a
+ * full launch waits for it; short launches at low load bound that wait,
+ * and once a backlog builds launches return to the slot capacity, where
+ * throughput is highest.  Results do not depend on it.  Takes effect from
+ * the next launch.  No counterpart in the reference. */
+int rm_server_set_lowload(rm_server *srv, int64_t launch_rows, int64_t backlog_rows);
 /* Seconds on the server's monotonic clock (the clock of the t_* stamps). */
 double rm_server_now(rm_server *srv);
 /* Stop the loop: requests not yet launched end as cancelled (or failed when

@@ -86,6 +86,9 @@ struct rm_server {
     std::vector<Slot *> free_slots;
     int64_t next_id = 0;
     int policy = RM_SCHED_FIFO;
+    // low-load launch cap (rm_server_set_lowload): while at most lowload_backlog
+    // rows wait unlaunched, a launch takes at most lowload_rows rows
+    int64_t lowload_rows = 0, lowload_backlog = 0;
     bool stop = false;
     bool disp_done = false;  // the dispatcher has exited: no launch will join inflight
     int64_t launches = 0, rows_done = 0, max_inflight = 0;
@@ -211,8 +214,17 @@ void dispatcher(rm_server *sv) {
         int64_t rows = 0;
         const double t = sv->now();
         bool emptied = false;
+        // launch size: the slot's capacity, or the low-load cap while the
+        // unlaunched backlog is small -- a launch then holds the GPU for a
+        // short time, so a request that arrives meanwhile waits less
+        int64_t cap = sv->max_rows;
+        if (sv->lowload_rows > 0) {
+            int64_t backlog = 0;
+            for (const Req *r : sv->queue) backlog += r->n - r->next_row;
+            if (backlog <= sv->lowload_backlog) cap = std::min(cap, sv->lowload_rows);
+        }
         auto take_rows = [&](Req *r, int64_t want) {
-            const int64_t take = std::min({r->n - r->next_row, sv->max_rows - rows, want});
+            const int64_t take = std::min({r->n - r->next_row, cap - rows, want});
             if (take <= 0) return;
             sl->pieces.push_back({r, r->next_row, r->next_row + take, rows});
             ++r->pending;
@@ -229,14 +241,14 @@ void dispatcher(rm_server *sv) {
         // (so a large request keeps progressing while small ones keep coming)
         std::vector<Req *> order(sv->queue.begin(), sv->queue.end());
         if (sv->policy == RM_SCHED_SRPT) {
-            take_rows(order.front(), std::max<int64_t>(1, sv->max_rows / 4));
+            take_rows(order.front(), std::max<int64_t>(1, cap / 4));
             std::stable_sort(order.begin(), order.end(), [](const Req *x, const Req *y) {
                 return x->n - x->next_row < y->n - y->next_row;
             });
         }
         for (Req *r : order) {
-            if (rows == sv->max_rows) break;
-            take_rows(r, sv->max_rows);
+            if (rows == cap) break;
+            take_rows(r, cap);
         }
         if (emptied)
             sv->queue.erase(std::remove_if(sv->queue.begin(), sv->queue.end(),
@@ -460,6 +472,16 @@ int rm_server_set_policy(rm_server *srv, int policy) {
              "policy must be RM_SCHED_FIFO or RM_SCHED_SRPT");
     std::lock_guard<std::mutex> lk(srv->mu);
     srv->policy = policy;
+    return RM_OK;
+}
+
+int rm_server_set_lowload(rm_server *srv, int64_t launch_rows, int64_t backlog_rows) {
+    RM_CHECK(srv, RM_ERR_ARG, "NULL argument");
+    RM_CHECK(launch_rows >= 0 && backlog_rows >= 0, RM_ERR_ARG,
+             "launch_rows and backlog_rows must be >= 0");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->lowload_rows = launch_rows;
+    srv->lowload_backlog = backlog_rows;
     return RM_OK;
 }
 

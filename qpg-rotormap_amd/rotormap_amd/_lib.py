@@ -82,6 +82,7 @@ SIGNATURES = {
     "rm_server_cancel": (i32, [vp, i64]),
     "rm_server_stats": (i32, [vp, vp, vp, vp]),
     "rm_server_set_policy": (i32, [vp, i32]),
+    "rm_server_set_lowload": (i32, [vp, i64, i64]),
     "rm_server_destroy": (i32, [vp]),
     "rm_fasta_reads": (i32, [vp, vp, i64, i32, vp, i64, vp, vp, C.POINTER(i64), vp]),
     "rm_fasta_reads_host": (i32, [vp, vp, i64, i32, vp, i64, vp, vp, C.POINTER(i64)]),

@@ -45,10 +45,12 @@ class Server:
     """rm_server over `index` for encoder `re`.  max_rows: rows per launch;
     n_streams: launch slots in flight; hbm_budget: byte cap on the slots'
     buffers (0 = the free HBM less 96 MiB); policy: "fifo" or "srpt", the
-    order in which queued rows fill a launch (set_policy changes it)."""
+    order in which queued rows fill a launch (set_policy changes it);
+    lowload: (launch_rows, backlog_rows) for set_lowload, or None."""
 
     def __init__(self, index, re: RopeEncoder, k=20, kprime=32, mode="fp8_rescore",
-                 max_rows=65536, n_streams=2, hbm_budget=0, device=0, policy="fifo"):
+                 max_rows=65536, n_streams=2, hbm_budget=0, device=0, policy="fifo",
+                 lowload=None):
         if not isinstance(index, ResidentIndex):
             index = ResidentIndex(index, fp16=(mode == "fp16"), device=device)
         self.index, self.re, self.k = index, re, int(k)
@@ -61,6 +63,17 @@ class Server:
         self._refs = {}
         self._mu = threading.Lock()
         self.set_policy(policy)
+        self.lowload = (0, 0)
+        if lowload:
+            self.set_lowload(*lowload)
+
+    def set_lowload(self, launch_rows, backlog_rows):
+        """While at most backlog_rows rows wait unlaunched, launches take at
+        most launch_rows rows (0 = off): shorter launches at low load bound
+        the wait of a request that arrives behind one; full launches once a
+        backlog builds keep the throughput."""
+        L.check(L.lib().rm_server_set_lowload(self.handle, int(launch_rows), int(backlog_rows)))
+        self.lowload = (int(launch_rows), int(backlog_rows))
 
     def set_policy(self, policy):
         """"fifo" (arrival order) or "srpt" (fewest remaining rows first)."""

@@ -378,7 +378,7 @@ def test_index_past_2_32_elements(torch_gpu):
 
 def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
     """With no tuned plan, a batch takes the smallest ring tile that holds it
-    (64 rows up to 64, 128 up to 128, else 256) and the smallest split count
+    (64 rows up to 64, 128 up to 256, else 256) and the smallest split count
     (from 'one item per CU' up, doubling) for which the query tiles worked on
     at once, min(R, CUs / S) tiles of that many rows, fit in 128 MB: a
     65,536-read e4m3 batch (2 KB rows, 128 MB) stays unsplit on 256 CUs; the
@@ -395,7 +395,7 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
                 L.check(lib.rm_plan_set(ctx, M, mode, 0, 0, 0))
 
         def expect(M, row_bytes, kp=20):
-            bq = 128 if kp > 32 else (64 if M <= 64 else 128 if M <= 128 else 256)
+            bq = 128 if kp > 32 else (64 if M <= 64 else 128 if M <= 256 else 256)
             rt = -(-M // bq)
             S = max(1, min(cus // rt, 256))
             while S < 256 and min(rt, -(-cus // S)) * bq * row_bytes > 128 << 20:
@@ -412,7 +412,7 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
         assert (p8, p16) == (expect(65_536, 2048), expect(65_536, 4096))
         if cus == 256:
             assert (p8[1], p16[1]) == (1, 2)
-        for M in (4_096, 100, 30):
+        for M in (4_096, 200, 100, 30):
             assert plan(M, L.RM_MODE_FP8) == expect(M, 2048), M
         assert plan(30, L.RM_MODE_FP8, kp=48) == expect(30, 2048, kp=48)
         # a pinned plan; a tile the list length cannot run falls back

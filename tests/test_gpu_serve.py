@@ -107,6 +107,28 @@ def test_server_srpt_matches_oneshot(torch_gpu, setup):
         assert r.t_submit <= r.t_admit <= r.t_finish
 
 
+def test_server_lowload_launch_cap(torch_gpu, setup):
+    """With the low-load cap, a launch takes at most launch_rows rows while
+    at most backlog_rows rows wait unlaunched: a 3,000-row request on
+    2,048-row slots runs as one full launch (backlog 3,000) and then 256-row
+    launches (backlog <= 1,000): 5 launches.  Without the cap it takes 2.
+    Rows equal the one-shot rows either way."""
+    ref, re, idx, reads = setup
+    v0, l0 = oneshot(torch_gpu, idx, re, reads.words, "fp8_rescore")
+    with R.Server(idx.embeds8, re, max_rows=2048, n_streams=2, policy="srpt",
+                  lowload=(256, 1000)) as srv:
+        out = srv.wait(*srv.submit(reads.words))
+        assert srv.stats()["launches"] == 5
+        srv.set_lowload(0, 0)
+        out2 = srv.wait(*srv.submit(reads.words))
+        assert srv.stats()["launches"] == 7
+        with pytest.raises(R.RotorMapArgumentError):
+            srv.set_lowload(-1, 0)
+    for r in (out, out2):
+        np.testing.assert_array_equal(r.vals, v0)
+        np.testing.assert_array_equal(r.locs, l0)
+
+
 def test_server_admission_and_errors(torch_gpu, setup):
     ref, re, idx, reads = setup
     B = R.ResidentIndex(idx.embeds8)
