@@ -457,8 +457,10 @@ int rm_rowtopk_merge(rm_ctx *ctx, float *D_val, int32_t *D_loc, int64_t ldd, con
     int rc;
     if ((rc = ws_get(ctx, "rowtopk_v", (size_t)M * G * k * 4, &pv))) return rc;
     if ((rc = ws_get(ctx, "rowtopk_c", (size_t)M * G * k * 4, &pc))) return rc;
+    void *seed;
+    if ((rc = ws_get(ctx, "rowtopk_seed", (size_t)M * 4, &seed))) return rc;
     return launch_rowtopk(D_val, D_loc, ldd, C, c_dtype, ldc, M, wi, offset, k, G, (float *)pv,
-                          (int32_t *)pc, pick_stream(ctx, stream));
+                          (int32_t *)pc, (int32_t *)seed, pick_stream(ctx, stream));
 }
 
 int rm_index_create(rm_ctx *ctx, const uint8_t *B, int64_t rdim, int64_t N, int flags,

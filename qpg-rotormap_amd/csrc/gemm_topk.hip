@@ -1327,7 +1327,9 @@ int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int6
 // segment lists and the running list (earlier columns) by a k-step
 // tournament under the total order (value desc, column asc), so the result
 // equals one sequential scan.  Lists of the segments go through a workspace
-// of M x G x k (value, column) pairs.
+// of M x G x k (value, column) pairs.  Before pass 1, rows whose running list
+// is not yet full (the first chunk) get a chunk-level admission bound
+// (rowtopk_bound_kernel).
 // ---------------------------------------------------------------------------
 template <int CDT>
 __device__ __forceinline__ float load_c(const void *C, int64_t o) {
@@ -1336,12 +1338,43 @@ __device__ __forceinline__ float load_c(const void *C, int64_t o) {
     else return e4m3_to_f32(((const uint8_t *)C)[o]);
 }
 
+// Pass 0, for rows whose running list is not full yet (the first chunk of a
+// batch): cut the chunk's columns into k groups; each group's maximum is a
+// distinct value of the chunk, so the smallest of the k maxima has at least k
+// values at or above it and bounds the k-th value of (running list + chunk)
+// from below.  Pass 1 then admits only values >= it (with column INT_MAX as
+// the threshold's column, equal values pass), which skips the insertions of
+// the segments' warm-up.  One wave per (64 rows, group); the bound is the
+// atomicMin of order keys over the groups (seed preset to INT_MAX).
+template <int CDT>
+__global__ __launch_bounds__(64) void rowtopk_bound_kernel(const void *C, int64_t ldc, int64_t M,
+                                                           int64_t wi, int k, const int32_t *dl,
+                                                           int64_t ldd, int32_t *seed) {
+    constexpr int U = 8;
+    const int64_t row = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int g = blockIdx.y;
+    if (row >= M || dl[row + (int64_t)(k - 1) * ldd] > 0) return;
+    const int64_t j0 = wi * g / k, j1 = wi * (g + 1) / k;
+    float gm = -INFINITY;
+    int64_t j = j0;
+    for (; j + U <= j1; j += U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = load_c<CDT>(C, row + (j + u) * ldc);
+#pragma unroll
+        for (int u = 0; u < U; ++u) gm = fmaxf(gm, v[u]);
+    }
+    for (; j < j1; ++j) gm = fmaxf(gm, load_c<CDT>(C, row + j * ldc));
+    atomicMin(&seed[row], order_key(gm));
+}
+
 template <int CDT>
 __global__ __launch_bounds__(64) void rowtopk_seg_kernel(const void *C, int64_t ldc,
                                                          int64_t M, int64_t wi, int64_t offset,
                                                          int k, int G, const float *dv,
                                                          const int32_t *dl, int64_t ldd,
-                                                         float *pv, int32_t *pc) {
+                                                         const int32_t *seed, float *pv,
+                                                         int32_t *pc) {
     constexpr int U = 8;  // columns per step: their loads are issued together
     extern __shared__ float lsh[];
     const int t = threadIdx.x;
@@ -1365,6 +1398,12 @@ __global__ __launch_bounds__(64) void rowtopk_seg_kernel(const void *C, int64_t 
             tv = dv[row + (int64_t)(k - 1) * ldd];
             tc = l - 1;
         }
+    }
+    // no full running list yet (the first chunk): the chunk-level bound of
+    // pass 0, when there is one
+    if (tc == INT_MAX && seed != nullptr) {
+        const int32_t key = seed[row];
+        if (key != INT_MAX) tv = order_key_inv(key);
     }
     auto insert = [&](float v, int32_t c) {
         int pos = k - 1;
@@ -1463,22 +1502,37 @@ int rowtopk_segments(int64_t M, int64_t wi) {
 
 int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dtype, int64_t ldc,
                    int64_t M, int64_t wi, int64_t offset, int k, int G, float *pv, int32_t *pc,
-                   hipStream_t st) {
+                   int32_t *seed, hipStream_t st) {
     if (M == 0) return RM_OK;
     RM_CHECK(G >= 1 && G <= 63, RM_ERR_ARG, "segments must be 1..63 (got %d)", G);
     RM_CHECK(cdiv(M, 64) <= 0x7fffffffLL && M <= 0x7fffffffLL && M * 64 <= kMaxWorkItems,
              RM_ERR_SHAPE, "%lld rows exceed one launch", (long long)M);
     const size_t lds = (size_t)64 * (2 * k + 1) * 4;
+    // pass 0 only where it can bind: k groups of at least 2 columns
+    if (seed != nullptr && wi >= 2 * (int64_t)k) {
+        fill_i32_kernel<<<dim3(stride_grid(M, 256)), dim3(256), 0, st>>>(seed, M, INT_MAX);
+        RM_HIP(hipGetLastError());
+        const dim3 g0((unsigned)cdiv(M, 64), (unsigned)k);
+        if (c_dtype == RM_F32)
+            rowtopk_bound_kernel<RM_F32><<<g0, dim3(64), 0, st>>>(C, ldc, M, wi, k, dl, ldd, seed);
+        else if (c_dtype == RM_F16)
+            rowtopk_bound_kernel<RM_F16><<<g0, dim3(64), 0, st>>>(C, ldc, M, wi, k, dl, ldd, seed);
+        else
+            rowtopk_bound_kernel<RM_E4M3><<<g0, dim3(64), 0, st>>>(C, ldc, M, wi, k, dl, ldd, seed);
+        RM_HIP(hipGetLastError());
+    } else {
+        seed = nullptr;
+    }
     const dim3 grid((unsigned)cdiv(M, 64), (unsigned)G);
     if (c_dtype == RM_F32)
         rowtopk_seg_kernel<RM_F32><<<grid, dim3(64), lds, st>>>(C, ldc, M, wi, offset, k, G, dv, dl,
-                                                               ldd, pv, pc);
+                                                               ldd, seed, pv, pc);
     else if (c_dtype == RM_F16)
         rowtopk_seg_kernel<RM_F16><<<grid, dim3(64), lds, st>>>(C, ldc, M, wi, offset, k, G, dv, dl,
-                                                               ldd, pv, pc);
+                                                               ldd, seed, pv, pc);
     else
         rowtopk_seg_kernel<RM_E4M3><<<grid, dim3(64), lds, st>>>(C, ldc, M, wi, offset, k, G, dv,
-                                                                dl, ldd, pv, pc);
+                                                                dl, ldd, seed, pv, pc);
     RM_HIP(hipGetLastError());
     rowtopk_final_kernel<<<dim3((unsigned)M), dim3(64), 0, st>>>(dv, dl, ldd, M, k, G, pv, pc);
     RM_HIP(hipGetLastError());

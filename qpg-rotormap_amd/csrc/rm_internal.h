@@ -177,11 +177,13 @@ int launch_merge(const float *part_v, const int32_t *part_c, int S, int KP, int6
                  int db_dtype, int64_t ldb, int K, float *vals, int32_t *locs, int64_t ldo,
                  hipStream_t st);
 // the running row top-k of a C chunk in two passes: G column segments per
-// chunk (rowtopk_segments), their lists in pv / pc (M x G x k each)
+// chunk (rowtopk_segments), their lists in pv / pc (M x G x k each); seed (M
+// ints, or NULL) holds the per-row bound of pass 0 for rows whose running
+// list is not full
 int rowtopk_segments(int64_t M, int64_t wi);
 int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dtype, int64_t ldc,
                    int64_t M, int64_t wi, int64_t offset, int k, int G, float *pv, int32_t *pc,
-                   hipStream_t st);
+                   int32_t *seed, hipStream_t st);
 int launch_transpose_u8(uint8_t *dst, const uint8_t *src, int64_t rows, int64_t cols,
                         int64_t s_row, int64_t s_col, int esize, hipStream_t st);
 

@@ -281,6 +281,23 @@ def test_rowtopk_merge_dropin(torch_gpu):
         ev, el = O.topk_rows(Call, k)
         np.testing.assert_array_equal(Dl.cpu().numpy(), el)
         np.testing.assert_array_equal(Dv.cpu().numpy(), ev)
+    # uneven chunks: a first chunk narrower than k leaves the running list
+    # part-full, so the next chunk's admission bound comes from its own
+    # columns while earlier entries exist; chunks narrower than 2k take none
+    Dv = torch.full((k, M), float("-inf"), device="cuda").t()
+    Dl = torch.zeros(k, M, dtype=torch.int32, device="cuda").t()
+    Call = np.zeros((M, 1000), dtype=np.float32)
+    lo = 0
+    for wi in (15, 500, 3, 39, 443):
+        buf = torch.zeros(wi, M, dtype=torch.float32, device="cuda").t()
+        R.fp8_gemm_hip(buf, dq, db, wi, b_col0=lo)
+        R.launch_seg_rowtopk_merge(Dv, Dl, buf, lo, k)
+        torch.cuda.synchronize()
+        Call[:, lo:lo + wi] = buf.cpu().numpy()
+        lo += wi
+    ev, el = O.topk_rows(Call, k)
+    np.testing.assert_array_equal(Dl.cpu().numpy(), el)
+    np.testing.assert_array_equal(Dv.cpu().numpy(), ev)
 
 
 @pytest.mark.parametrize("M,k", [(100, 64), (700, 20), (64, 1)])
