@@ -213,8 +213,9 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words,
  * reference's fixed chunk width w and rows_cap (RotorMap/search/
  * engine_fp8.jl:309-337) are what it replaces.  Without a tuned entry a
  * batch takes 64 rows up to 64, 128 up to 256, else 256, and the
- * built-in split rule; lists longer than 32 (kp > 32) always run 128-row
- * tiles.  Results never depend on the plan.
+ * built-in split rule, stepping the tile down while a split would sweep
+ * fewer than 32 column tiles of the index; lists longer than 32 (kp > 32)
+ * always run 128-row tiles.  Results never depend on the plan.
  * rm_plan_get: the plan a batch of M rows gets; kp is the list length the
  * fused kernel keeps (kprime for RM_MODE_FP8_RESCORE, else ktop). */
 int rm_plan_get(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int32_t *tile_rows,

@@ -338,9 +338,12 @@ __device__ __forceinline__ void ring_wait(int wid) {
 // other's MFMAs, where in lockstep both read right after the barrier and the
 // MFMA pipe idles.  Same registers (the fragments live across the barrier),
 // same barriers and buffer lifetimes: every read of a stage still completes
-// before the barrier that precedes the refill of its buffer.
+// before the barrier that precedes the refill of its buffer.  Off by
+// default: alternated runs on one MI355X measured it neutral at the
+// human-scale index (332.6 / 332.6 ms against 330.2 / 334.8 ms per
+// 65,536-read launch) and 2.7 % slower at 100 Mbp (13.11 against 12.75 ms).
 #ifndef RM_LAG
-#define RM_LAG 1
+#define RM_LAG 0
 #endif
 // RM_SETPRIO: raise the wave priority while it issues a block of MFMAs
 #ifndef RM_SETPRIO

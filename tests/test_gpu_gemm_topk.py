@@ -400,7 +400,9 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
     at once, min(R, CUs / S) tiles of that many rows, fit in 128 MB: a
     65,536-read e4m3 batch (2 KB rows, 128 MB) stays unsplit on 256 CUs; the
     same batch in fp16 (4 KB rows, 256 MB) takes two splits; a 4,096-read
-    batch takes CUs / 16."""
+    batch takes CUs / 16.  While a split would sweep fewer than 32 of the
+    index's column tiles the tile halves (down to 64 rows): on this
+    512-tile index 100 and 200 reads run 64-row tiles."""
     torch = torch_gpu
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     idx = R.ResidentIndex(np.zeros((512 * 256, 2048), np.uint8), fp16=True)
@@ -411,12 +413,21 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
             for M in (65_536, 4_096, 100, 30):
                 L.check(lib.rm_plan_set(ctx, M, mode, 0, 0, 0))
 
-        def expect(M, row_bytes, kp=20):
-            bq = 128 if kp > 32 else (64 if M <= 64 else 128 if M <= 256 else 256)
+        tiles = 512  # column tiles of the index
+
+        def splits(M, bq, row_bytes):
             rt = -(-M // bq)
             S = max(1, min(cus // rt, 256))
             while S < 256 and min(rt, -(-cus // S)) * bq * row_bytes > 128 << 20:
                 S = min(2 * S, 256)
+            return S
+
+        def expect(M, row_bytes, kp=20):
+            bq = 128 if kp > 32 else (64 if M <= 64 else 128 if M <= 256 else 256)
+            S = splits(M, bq, row_bytes)
+            while kp <= 32 and bq > 64 and -(-tiles // S) < 32:
+                bq //= 2
+                S = splits(M, bq, row_bytes)
             return bq, S, 0
 
         def plan(M, mode, kp=20):
@@ -431,6 +442,8 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
             assert (p8[1], p16[1]) == (1, 2)
         for M in (4_096, 200, 100, 30):
             assert plan(M, L.RM_MODE_FP8) == expect(M, 2048), M
+        if cus == 256:
+            assert plan(200, L.RM_MODE_FP8)[0] == 64 and plan(4_096, L.RM_MODE_FP8)[0] == 256
         assert plan(30, L.RM_MODE_FP8, kp=48) == expect(30, 2048, kp=48)
         # a pinned plan; a tile the list length cannot run falls back
         L.check(lib.rm_plan_set(ctx, 4_096, L.RM_MODE_FP8, 64, 8, 2))
