@@ -304,13 +304,17 @@ int rm_server_stats(rm_server *srv, int64_t *launches, int64_t *rows_done, int64
 #define RM_SCHED_FIFO 0
 #define RM_SCHED_SRPT 1
 int rm_server_set_policy(rm_server *srv, int policy);
-/* Low-load launch cap: while at most backlog_rows rows wait unlaunched, a
- * launch takes at most launch_rows rows (0 = off, the default).  The fused
- * kernel holds every CU while it runs, so a request that arrives behind a
- * full launch waits for it; short launches at low load bound that wait,
- * and once a backlog builds launches return to the slot capacity, where
- * throughput is highest.  Results do not depend on it.  Takes effect from
- * the next launch.  No counterpart in the reference. */
+/* Low-load mode (0 = off, the default): while at most backlog_rows rows wait
+ * unlaunched, (1) one launch is in flight at a time, (2) a launch holds whole
+ * requests only, in the policy's order, while they fit in launch_rows rows,
+ * and (3) a first request larger than that gets a launch_rows slice.  The
+ * fused kernel holds every CU while it runs, so a request that arrives
+ * during a large one otherwise waits for the running launch, the launch
+ * queued behind it and its own launch padded with the large request's rows.
+ * Once a backlog builds the server returns to full, padded launches with
+ * every slot in flight, where throughput is highest.  Results do not depend
+ * on it.  Takes effect from the next launch.  No counterpart in the
+ * reference. */
 int rm_server_set_lowload(rm_server *srv, int64_t launch_rows, int64_t backlog_rows);
 /* Seconds on the server's monotonic clock (the clock of the t_* stamps). */
 double rm_server_now(rm_server *srv);

@@ -86,8 +86,9 @@ struct rm_server {
     std::vector<Slot *> free_slots;
     int64_t next_id = 0;
     int policy = RM_SCHED_FIFO;
-    // low-load launch cap (rm_server_set_lowload): while at most lowload_backlog
-    // rows wait unlaunched, a launch takes at most lowload_rows rows
+    // low-load mode (rm_server_set_lowload): while at most lowload_backlog
+    // rows wait unlaunched, one launch is in flight at a time and it holds
+    // whole requests only, at most lowload_rows rows
     int64_t lowload_rows = 0, lowload_backlog = 0;
     bool stop = false;
     bool disp_done = false;  // the dispatcher has exited: no launch will join inflight
@@ -202,27 +203,33 @@ void fail_pieces(rm_server *sv, Slot *sl, int code, const std::string &msg) {
     }
 }
 
+// low-load mode: a small backlog of unlaunched rows (caller holds sv->mu)
+bool low_load(const rm_server *sv) {
+    if (sv->lowload_rows <= 0) return false;
+    int64_t backlog = 0;
+    for (const Req *r : sv->queue) backlog += r->n - r->next_row;
+    return backlog <= sv->lowload_backlog;
+}
+
 void dispatcher(rm_server *sv) {
     hipSetDevice(sv->ctx->device);
     std::unique_lock<std::mutex> lk(sv->mu);
     for (;;) {
-        sv->cv.wait(lk, [&] { return sv->stop || (!sv->queue.empty() && !sv->free_slots.empty()); });
+        // at low load a launch waits for the GPU to be idle, so a request
+        // that arrives meanwhile is not queued behind a second launch
+        sv->cv.wait(lk, [&] {
+            return sv->stop || (!sv->queue.empty() && !sv->free_slots.empty() &&
+                                (sv->inflight.empty() || !low_load(sv)));
+        });
         if (sv->stop) break;
+        const bool low = low_load(sv);
         Slot *sl = sv->free_slots.back();
         sv->free_slots.pop_back();
         sl->pieces.clear();
         int64_t rows = 0;
         const double t = sv->now();
         bool emptied = false;
-        // launch size: the slot's capacity, or the low-load cap while the
-        // unlaunched backlog is small -- a launch then holds the GPU for a
-        // short time, so a request that arrives meanwhile waits less
-        int64_t cap = sv->max_rows;
-        if (sv->lowload_rows > 0) {
-            int64_t backlog = 0;
-            for (const Req *r : sv->queue) backlog += r->n - r->next_row;
-            if (backlog <= sv->lowload_backlog) cap = std::min(cap, sv->lowload_rows);
-        }
+        const int64_t cap = low ? std::min(sv->max_rows, sv->lowload_rows) : sv->max_rows;
         auto take_rows = [&](Req *r, int64_t want) {
             const int64_t take = std::min({r->n - r->next_row, cap - rows, want});
             if (take <= 0) return;
@@ -240,15 +247,31 @@ void dispatcher(rm_server *sv) {
         // after the oldest request has taken up to a quarter of the launch
         // (so a large request keeps progressing while small ones keep coming)
         std::vector<Req *> order(sv->queue.begin(), sv->queue.end());
-        if (sv->policy == RM_SCHED_SRPT) {
-            take_rows(order.front(), std::max<int64_t>(1, cap / 4));
+        auto srpt_sort = [&] {
             std::stable_sort(order.begin(), order.end(), [](const Req *x, const Req *y) {
                 return x->n - x->next_row < y->n - y->next_row;
             });
-        }
-        for (Req *r : order) {
-            if (rows == cap) break;
-            take_rows(r, cap);
+        };
+        if (low) {
+            // whole requests in the policy's order while they fit; a first
+            // request larger than the cap gets a slice of it.  A small
+            // request then rides a short launch of its own instead of one
+            // padded with a large request's rows.
+            if (sv->policy == RM_SCHED_SRPT) srpt_sort();
+            for (Req *r : order) {
+                const int64_t left = r->n - r->next_row;
+                if (rows > 0 && left > cap - rows) break;
+                take_rows(r, cap);
+            }
+        } else {
+            if (sv->policy == RM_SCHED_SRPT) {
+                take_rows(order.front(), std::max<int64_t>(1, cap / 4));
+                srpt_sort();
+            }
+            for (Req *r : order) {
+                if (rows == cap) break;
+                take_rows(r, cap);
+            }
         }
         if (emptied)
             sv->queue.erase(std::remove_if(sv->queue.begin(), sv->queue.end(),

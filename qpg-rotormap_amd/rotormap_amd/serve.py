@@ -68,10 +68,11 @@ class Server:
             self.set_lowload(*lowload)
 
     def set_lowload(self, launch_rows, backlog_rows):
-        """While at most backlog_rows rows wait unlaunched, launches take at
-        most launch_rows rows (0 = off): shorter launches at low load bound
-        the wait of a request that arrives behind one; full launches once a
-        backlog builds keep the throughput."""
+        """Low-load mode (0 = off): while at most backlog_rows rows wait
+        unlaunched, one launch is in flight at a time and it holds whole
+        requests only (a request larger than launch_rows gets a slice), so a
+        small request that arrives during a large one waits at most for the
+        running launch; full, padded launches return once a backlog builds."""
         L.check(L.lib().rm_server_set_lowload(self.handle, int(launch_rows), int(backlog_rows)))
         self.lowload = (int(launch_rows), int(backlog_rows))
 

@@ -108,11 +108,11 @@ def test_server_srpt_matches_oneshot(torch_gpu, setup):
 
 
 def test_server_lowload_launch_cap(torch_gpu, setup):
-    """With the low-load cap, a launch takes at most launch_rows rows while
-    at most backlog_rows rows wait unlaunched: a 3,000-row request on
+    """Low-load mode: while at most backlog_rows rows wait unlaunched a launch
+    holds at most launch_rows rows, one at a time: a 3,000-row request on
     2,048-row slots runs as one full launch (backlog 3,000) and then 256-row
-    launches (backlog <= 1,000): 5 launches.  Without the cap it takes 2.
-    Rows equal the one-shot rows either way."""
+    slices (backlog <= 1,000): 5 launches.  Without it, 2.  Rows equal the
+    one-shot rows either way."""
     ref, re, idx, reads = setup
     v0, l0 = oneshot(torch_gpu, idx, re, reads.words, "fp8_rescore")
     with R.Server(idx.embeds8, re, max_rows=2048, n_streams=2, policy="srpt",
@@ -127,6 +127,28 @@ def test_server_lowload_launch_cap(torch_gpu, setup):
     for r in (out, out2):
         np.testing.assert_array_equal(r.vals, v0)
         np.testing.assert_array_equal(r.locs, l0)
+
+
+def test_server_lowload_whole_requests(torch_gpu, setup):
+    """Low-load mode packs whole requests: small requests submitted behind a
+    large one ride launches of their own and finish before it, with one
+    launch in flight at a time; every row equals its one-shot row."""
+    ref, re, idx, reads = setup
+    v0, l0 = oneshot(torch_gpu, idx, re, reads.words, "fp8_rescore")
+    spans = [(0, 2500), (2500, 2800), (2800, 2810), (2810, 3000)]
+    with R.Server(idx.embeds8, re, max_rows=1024, n_streams=2, policy="srpt",
+                  lowload=(512, 1 << 20)) as srv:
+        ids = [srv.submit(reads.words[a:b]) for a, b in spans]
+        out = [srv.wait(rid, n) for rid, n in ids]
+        st = srv.stats()
+    assert st["max_inflight"] == 1
+    assert st["launches"] >= 5 + 1  # the large one alone needs 5 slices of 512
+    big_done = out[0].t_finish
+    for r in out[1:]:
+        assert r.t_finish < big_done
+    for (a, b), r in zip(spans, out):
+        np.testing.assert_array_equal(r.vals, v0[a:b])
+        np.testing.assert_array_equal(r.locs, l0[a:b])
 
 
 def test_server_admission_and_errors(torch_gpu, setup):
