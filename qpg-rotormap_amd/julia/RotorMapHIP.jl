@@ -99,7 +99,7 @@ const POLICIES = Dict(:fifo => Cint(0), :srpt => Cint(1))  # RM_SCHED_FIFO, RM_S
 function HipServer(index::HipIndex, kmer::Integer, s::Integer, m::Integer, c::Integer;
                    k::Integer = 20, kprime::Integer = 32, mode::Symbol = :fp8_rescore,
                    max_rows::Integer = 2^14, n_streams::Integer = 2, hbm_budget::Integer = 0,
-                   policy::Symbol = :fifo)
+                   policy::Symbol = :fifo, lowload::Tuple{Integer,Integer} = (0, 0))
     haskey(MODES, mode) || throw(ArgumentError("mode must be :fp8, :fp8_rescore or :fp16"))
     haskey(POLICIES, policy) || throw(ArgumentError("policy must be :fifo or :srpt"))
     out = Ref{Ptr{Cvoid}}(C_NULL)
@@ -113,6 +113,10 @@ function HipServer(index::HipIndex, kmer::Integer, s::Integer, m::Integer, c::In
                     v.ptr = C_NULL), srv)
     # :srpt fills each launch with the requests that have the fewest rows left
     check(ccall((:rm_server_set_policy, LIB), Cint, (Ptr{Cvoid}, Cint), srv.ptr, POLICIES[policy]))
+    # lowload = (launch_rows, backlog_rows): while the unlaunched backlog is
+    # small, one launch in flight holding whole requests (see the header)
+    check(ccall((:rm_server_set_lowload, LIB), Cint, (Ptr{Cvoid}, Int64, Int64), srv.ptr,
+                lowload[1], lowload[2]))
     return srv
 end
 
