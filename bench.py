@@ -44,6 +44,9 @@ Legs beside the headline step:
   fasta             the batch as FASTA text, parsed and 2-bit packed on the GPU
   cpu_baseline      the oracle's golden path (FP64 encoder, FP64 dots, top-20)
                     on all host cores, compared with the GPU's plain FP8 rows
+  cfg1_plain_1mbp   the 1 Mbp plain configuration (1000 exact + 200 mutated reads)
+                    in full on the CPU golden path and in every GPU mode: recall
+                    against the selftest floors, reads/s
 
 python bench.py [--workload human|100mbp] [--steps K] [--warmup W] [--gpus 1]
                 [--reads M] [--no-cpu] [--no-serve] [--no-uneven] [--no-fasta]
@@ -173,6 +176,8 @@ def main():
     ap.add_argument("--fp16-steps", type=int, default=1)
     ap.add_argument("--cpu-reads", type=int, default=0, help="0 = size the CPU sample to ~15 s")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cfg1", action="store_true",
+                    help="skip the 1 Mbp plain configuration run in full on the CPU and the GPU")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-fasta", action="store_true")
@@ -745,6 +750,56 @@ def main():
                         "FP64 (golden), the GPU in fixed point + fp32"},
             "host": lscpu_summary()}
         log(f"cpu vs gpu plain fp8: {result['cpu_baseline']['vs_gpu_plain_fp8']}")
+
+        # ---- the 1 Mbp plain configuration in full: CPU golden path and GPU ----
+        if not args.no_cfg1:
+            ref1 = R.synth_reference([600_000, 400_000], gc=0.5, seed=2024)
+            re1 = R.RopeEncoder(k=K_FRAG, s=8, m=4, c=4)
+            idx1 = R.build_index(ref1, re1, kstep=KSTEP)
+            ex = R.sample_reads(ref1, 1000, K_FRAG, err=0.0, seed=42, draw="random")
+            mu = R.sample_reads(ref1, 200, K_FRAG, err=0.05, seed=43, draw="random", model=0)
+            w1 = np.ascontiguousarray(np.concatenate([ex.words, mu.words]))
+            recs1 = np.concatenate([ex.recs, mu.recs])
+            starts1 = np.concatenate([ex.starts, mu.starts])
+            n1 = len(recs1)
+
+            def rec1(locs):
+                out = {}
+                for name, sl in (("err0", slice(0, 1000)), ("err005", slice(1000, n1))):
+                    r = R.count_mapped(locs[sl], recs1[sl], starts1[sl], idx1.rec_of_col,
+                                       idx1.starts, K_FRAG)
+                    out[name] = f"{r['correct']}/{r['total']}"
+                return out
+
+            t = time.perf_counter()
+            cv1, cl1 = O.cpu_map(w1, K_FRAG, 8, 4, 4, np.ascontiguousarray(idx1.embeds8), 20,
+                                 threads=cores)
+            tc1 = time.perf_counter() - t
+            dw1 = torch.from_numpy(w1.view(np.int32).reshape(-1).copy()).to(dev)
+            cfg1 = {"reference": "2 records (600 + 400 kbp), s8m4c4, k=20000 kstep=2000, "
+                                 f"N={idx1.embeds8.shape[0]} columns; 1000 exact reads + 200 at "
+                                 "5 % substitutions",
+                    "cpu": {"reads_per_s": rnd(n1 / tc1, 1), "cores": cores, "recall": rec1(cl1),
+                            "seconds": rnd(tc1, 2), "kind": "port (FP64 golden encoder, FP64 dots "
+                            "of the e4m3 query against the e4m3 columns, top-20)"},
+                    "gpu": {}}
+            for mode in ("fp16", "fp8_rescore", "fp8"):
+                e1 = R.TopKEngine(idx1.embeds8, k=20, kprime=32, mode=mode, rows_cap=n1)
+                e1.map_words(re1, dw1, n1)
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                for _ in range(10):
+                    v1, l1 = e1.map_words(re1, dw1, n1)
+                torch.cuda.synchronize()
+                el1 = (time.perf_counter() - t) / 10
+                l1 = l1.cpu().numpy()
+                cfg1["gpu"][mode] = {"reads_per_s": rnd(n1 / el1, 1), "recall": rec1(l1)}
+                if mode == "fp8":
+                    cfg1["gpu"][mode]["same_top20_set_as_cpu"] = rnd(np.mean(
+                        [set(a.tolist()) == set(b.tolist()) for a, b in zip(l1, cl1)]), 4)
+            cfg1["floors"] = "err 0: 1000/1000; err 0.05: >= 180/200 (the reference's selftest)"
+            result["cfg1_plain_1mbp"] = cfg1
+            log(f"1 Mbp plain config: {cfg1}")
     print(json.dumps(result), flush=True)
 
 
