@@ -529,3 +529,30 @@ def test_rescore_on_verbatim_fp16_columns(torch_gpu):
     # an e4m3-only index has no fp16 columns to rescore against
     with pytest.raises(L.RotorMapError):
         R.ResidentIndex(B8).set_rescore("fp16")
+
+
+@pytest.mark.parametrize("K,kp", [(512, 32), (2048, 48), (320, 20)])
+def test_rescore_general_path_bitexact(torch_gpu, K, kp):
+    """The rescoring kernel's general path (k' > 32, or K not a multiple of
+    1,024: per-candidate gathers through LDS) against the oracle's fixed
+    order, on e4m3 columns and on an fp16 index's verbatim columns."""
+    torch = torch_gpu
+    M, N, k = 96, 1800, 20
+    Q8, Q32 = unit_rows_e4m3(M, K, 90 + K)
+    rng = np.random.default_rng(91 + kp)
+    x = rng.standard_normal((N, K))
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    B16 = x.astype(np.float16)
+    B8 = O.f32_to_e4m3(B16.astype(np.float32).ravel()).reshape(N, K)
+    _, cand = O.topk_rows(gemm_c(torch, Q8, B8), kp)
+    for cols in ("e4m3", "fp16"):
+        idx = R.ResidentIndex(B16)
+        idx.set_rescore(cols)
+        eng = R.TopKEngine(idx, k=k, kprime=kp, mode="fp8_rescore")
+        v, l = fused(torch, eng, Q8, Q32)
+        Bc = B16 if cols == "fp16" else B8
+        for r in range(M):
+            ev, el = O.rescore_rank(Q32[r], Bc, cand[r], k)
+            np.testing.assert_array_equal(l[r], el, err_msg=f"{cols} row {r}")
+            np.testing.assert_array_equal(v[r], ev, err_msg=f"{cols} row {r}")
+        idx.close()
