@@ -66,8 +66,8 @@ def test_julia_glue_calls_declared_functions(path):
 
 def test_header_and_integration_example_compile_as_c(tmp_path):
     """The header is plain C (C callers and FFI generators read it), and the C
-    example in INTEGRATION.md compiles against it with no implicit
-    declarations or pointer mismatches."""
+    example of the repository's integration guide compiles against it with no
+    implicit declarations or pointer mismatches."""
     import shutil
     import subprocess
     gcc = shutil.which("gcc")
@@ -75,7 +75,7 @@ def test_header_and_integration_example_compile_as_c(tmp_path):
         pytest.skip("no gcc")
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     blocks = re.findall(r"```c\n(.*?)```", text, re.S)
-    assert blocks, "INTEGRATION.md has no C example"
+    assert blocks, "the integration guide has no C example"
     body = "\n".join(l for l in blocks[0].splitlines() if not l.startswith("#include"))
     src = tmp_path / "example.c"
     src.write_text("#include <stddef.h>\n#include <stdint.h>\n#include \"rotormap_hip.h\"\n"
