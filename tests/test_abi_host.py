@@ -107,6 +107,12 @@ def test_null_arguments_are_rejected_without_a_device():
                              None) == L.RM_ERR_ARG
     assert lib.rm_server_wait(None, 0, 0, None, None, None, None, None) == L.RM_ERR_ARG
     assert lib.rm_server_set_policy(None, L.RM_SCHED_SRPT) == L.RM_ERR_ARG
+    assert lib.rm_server_set_lowload(None, 4096, 16384) == L.RM_ERR_ARG
+    assert lib.rm_index_set_rescore(None, L.RM_F16) == L.RM_ERR_ARG
+    assert lib.rm_index_rescore(None) == -1
+    assert lib.rm_rowtopk_merge(None, None, None, 1, None, L.RM_F32, 1, 1, 1, 0, 20,
+                                None) == L.RM_ERR_ARG
+    assert lib.rm_plan_set(None, 100, L.RM_MODE_FP8, 64, 0, 0) == L.RM_ERR_ARG
     assert lib.rm_server_destroy(None) == L.RM_OK
     assert lib.rm_index_cols(None) == 0
     assert lib.rm_device_free_bytes(None) == -1
