@@ -1508,7 +1508,10 @@ int launch_rowtopk(float *dv, int32_t *dl, int64_t ldd, const void *C, int c_dty
                    int32_t *seed, hipStream_t st) {
     if (M == 0) return RM_OK;
     RM_CHECK(G >= 1 && G <= 63, RM_ERR_ARG, "segments must be 1..63 (got %d)", G);
-    RM_CHECK(cdiv(M, 64) <= 0x7fffffffLL && M <= 0x7fffffffLL && M * 64 <= kMaxWorkItems,
+    // work-items per launch: the final pass M x 64, pass 1 (64 rows per wave)
+    // x G segments, pass 0 x k groups
+    RM_CHECK(cdiv(M, 64) <= 0x7fffffffLL && M <= 0x7fffffffLL && M * 64 <= kMaxWorkItems &&
+                 cdiv(M, 64) * 64 * std::max(G, k) <= kMaxWorkItems,
              RM_ERR_SHAPE, "%lld rows exceed one launch", (long long)M);
     const size_t lds = (size_t)64 * (2 * k + 1) * 4;
     // pass 0 only where it can bind: k groups of at least 2 columns
