@@ -12,8 +12,8 @@ from .encode import (RopeEncoder, dim, encode_frag_real_batch, encode_frag_real_
 from .gemm import f8_to_f32, f32_to_f8, fp8_gemm_hip, launch_seg_rowtopk_merge
 from .fasta import fasta_reads_gpu, iter_fasta_gpu
 from .genome import Reference, Record, pack_codes, read_fasta, synth_reference, unpack_words, write_fasta
-from .index import (IndexData, build_index, load_index, load_index_dir, save_index,
-                    save_index_dir, window_table)
+from .index import (IndexData, build_index, load_index, load_index_bin, load_index_dir, save_index,
+                    save_index_bin, save_index_dir, window_table)
 from .reads import ReadSet, even_alloc, parse_read_head, sample_reads
 from .serve import Server, ServedRequest, poisson_trace, run_trace
 from .search import MODES, ResidentIndex, TopKBatch, TopKEngine, count_mapped

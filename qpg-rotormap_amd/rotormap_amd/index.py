@@ -223,16 +223,10 @@ def load_index_dir(path, device=None):
             raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "embeds8.u8 size mismatch")
         e8 = e8.reshape(N, rd)
     else:
-        import torch
         e16 = np.fromfile(os.path.join(path, "embeds16.f16"), dtype=np.float16)
         if e16.size != N * rd:
             raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "embeds16.f16 size mismatch")
-        from .gemm import f32_to_f8
-        dev = torch.device(f"cuda:{device or 0}")
-        src = torch.from_numpy(e16).to(dev)
-        dst = torch.empty(N * rd, dtype=torch.uint8, device=dev)
-        f32_to_f8(dst, src)
-        e8 = dst.cpu().numpy().reshape(N, rd)
+        e8 = _quantize_rows(e16.reshape(N, rd), device)
     m_, n_ = meta.get("norms_shape", [meta["m"], N])
     norms = np.fromfile(os.path.join(path, "norms.f32"), dtype=np.float32)
     norms = norms.reshape(n_, m_) if m_ == meta["m"] else norms.reshape(m_, n_).T
@@ -246,3 +240,110 @@ def load_index_dir(path, device=None):
     return IndexData(e8, norms, heads, starts, strand, int(meta["k"]), int(meta["kstep"]),
                      int(meta["s"]), int(meta["m"]), int(meta["c"]), 0, int(meta["n_frag"]),
                      meta.get("source", ""), FP8_FORMAT, rec, names)
+
+
+def _quantize_rows(rows, device=None):
+    """(N, rdim) fp16 or fp32 rows -> (N, rdim) e4m3 bytes on the GPU, the
+    quantisation of index_to_f8 (RotorMap/index/load.jl:76-94): e4m3 of the
+    fp32 value (fp16 rows widen exactly first)."""
+    import torch
+    from .gemm import f32_to_f8
+    dev = torch.device(f"cuda:{device or 0}")
+    src = torch.from_numpy(np.ascontiguousarray(rows)).to(dev).reshape(-1)
+    dst = torch.empty(src.numel(), dtype=torch.uint8, device=dev)
+    f32_to_f8(dst, src)
+    return dst.cpu().numpy().reshape(rows.shape)
+
+
+FP8_BIN_FORMATS = (FP8_FORMAT, "indexflowreal.fp8.v1")
+ROW_BIN_FORMATS = ("indexreal.v1", "indexflowreal.v1")
+
+
+def save_index_bin(d, path, layout="fp8", source=None):
+    """Write `d` as RotorMap's index .bin: one Julia-serialized NamedTuple with
+    the fields, order and types of `_index_data_nt` (RotorMap/index/
+    build.jl:755-768).  layout "fp8" writes indexreal.fp8.v1 (embeds8, the
+    (rdim, N) e4m3 matrix); "fp16" writes indexreal.v1 with the verbatim fp16
+    rows (`d.embeds16`, (N, rdim)), which needs an index built with keep_f16."""
+    from . import binfmt as J
+    N, rd = d.embeds8.shape
+    if N != 2 * d.n_frag:
+        raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "embeds8 rows != 2 * n_frag")
+    if not (len(d.heads) == len(d.starts) == len(d.strand) == N):
+        raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "meta length mismatch")
+    norms = np.ascontiguousarray(d.norms, dtype=np.float32)   # (N, m) = Julia (m, N)
+    src = d.source if source is None else source
+    fields = [("format", FP8_FORMAT if layout == "fp8" else "indexreal.v1"), ("source", src),
+              ("k", int(d.k)), ("kstep", int(d.kstep)), ("s", int(d.s)), ("m", int(d.m)),
+              ("c", int(d.c)), ("normalize", int(d.normalize))]
+    et = {"norms": J.jtype("Float32"), "heads": J.jtype("String"), "starts": J.jtype("Int64"),
+          "strand": J.jtype("UInt8")}
+    if layout == "fp8":
+        fields += [("fp8", True), ("n_frag", int(d.n_frag)),
+                   ("embeds8", np.ascontiguousarray(d.embeds8, dtype=np.uint8).T)]
+        et["embeds8"] = J.jtype("Float8_E4M3FN")
+    elif layout == "fp16":
+        if d.embeds16 is None:
+            raise L.RotorMapArgumentError(L.RM_ERR_ARG, "layout fp16 needs the fp16 rows (keep_f16)")
+        fields += [("fp16", True), ("n_frag", int(d.n_frag)),
+                   ("embeds", np.asfortranarray(d.embeds16, dtype=np.float16))]
+        et["embeds"] = J.jtype("Float16")
+    else:
+        raise L.RotorMapArgumentError(L.RM_ERR_ARG, f"unknown layout {layout!r}")
+    fields += [("norms", norms.T), ("heads", [str(h) for h in d.heads]),
+               ("starts", np.asarray(d.starts, dtype=np.int64)),
+               ("strand", np.asarray(d.strand, dtype=np.uint8))]
+    J.jls_dump_namedtuple(path, fields, et)
+
+
+def load_index_bin(path, device=None):
+    """Read RotorMap's index .bin (a Julia-serialized NamedTuple) without
+    Julia, with the checks of load_index (RotorMap/index/load.jl:49-73).
+
+    indexreal.fp8.v1: embeds8 is a zero-copy view of the memory-mapped file.
+    indexreal.v1: the (N, rdim) rows are kept (`embeds16` when fp16) and
+    quantised to the e4m3 database on the GPU, as index_database does
+    (RotorMap/index/load.jl:156-161)."""
+    from . import binfmt as J
+    d, _ = J.jls_load(path)
+    if not isinstance(d, J.JNamedTuple):
+        raise L.RotorMapArgumentError(L.RM_ERR_ARG, f"not an index NamedTuple: {path}")
+    fmt = d.get("format")
+    if fmt not in FP8_BIN_FORMATS + ROW_BIN_FORMATS:
+        raise L.RotorMapArgumentError(L.RM_ERR_ARG,
+                                      f"not an indexreal.v1/indexreal.fp8.v1 index: {path}")
+    if d["normalize"] != 0:
+        raise L.RotorMapArgumentError(
+            L.RM_ERR_ARG, f"the mapping flow assumes a normalize = 0 index (got {d['normalize']})")
+    k, kstep, s, m, c, n_frag = (int(d[x]) for x in ("k", "kstep", "s", "m", "c", "n_frag"))
+    rd = 2 * m * 4 ** c
+    N = 2 * n_frag
+    types = dict(zip(d.keys(), d.types)) if hasattr(d, "types") else {}
+    emb16 = None
+    if fmt in FP8_BIN_FORMATS:
+        t = types.get("embeds8")
+        if t is not None and t != J.array_type(J.jtype("Float8_E4M3FN"), 2):
+            raise L.RotorMapArgumentError(L.RM_ERR_ARG, f"embeds8 eltype mismatch ({t!r})")
+        e8 = d["embeds8"]
+        if e8.shape != (rd, N):
+            raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "embeds8 shape mismatch")
+        e8 = e8.T
+    else:
+        rows = d["embeds"]
+        want = np.float16 if d.get("fp16") else np.float32
+        if rows.dtype != want:
+            raise L.RotorMapArgumentError(L.RM_ERR_ARG, "embeds eltype mismatch")
+        if rows.shape != (N, rd):
+            raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "embeds shape mismatch")
+        rows = np.ascontiguousarray(rows)
+        if rows.dtype == np.float16:
+            emb16 = rows
+        e8 = _quantize_rows(rows, device)
+    heads, starts, strand = d["heads"], np.asarray(d["starts"]), np.asarray(d["strand"])
+    if not (len(heads) == len(starts) == len(strand) == N):
+        raise L.RotorMapArgumentError(L.RM_ERR_SHAPE, "meta length mismatch")
+    norms = np.asarray(d["norms"])
+    norms = norms.T if norms.shape == (m, N) else norms
+    names, rec = _names_and_recs(heads)
+    return IndexData(e8, norms, list(heads), starts, strand, k, kstep, s, m, c, 0, n_frag,
+                     str(d.get("source", "")), FP8_FORMAT, rec, names, emb16)

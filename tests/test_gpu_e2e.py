@@ -106,3 +106,22 @@ def test_tiny_fp16_verbatim_index(torch_gpu, tiny):
         v, l = eng.map_words(re, dw, reads.words.shape[0])
         torch_gpu.cuda.synchronize()
         assert score(idx, reads, l.cpu().numpy())["correct"] >= floor
+
+
+def test_index_bin_round_trip_maps_the_same(torch_gpu, tiny, tmp_path):
+    """The index saved as the reference's .bin (both layouts) loads back to the
+    same e4m3 database: indexreal.fp8.v1 verbatim, indexreal.v1 (fp16 rows)
+    through the GPU quantiser, e4m3(fp32(fp16)) = the build's own columns; the
+    loaded index maps the reads to the same rows."""
+    ref, re, idx, exact, _ = tiny
+    R.save_index_bin(idx, tmp_path / "i8.bin")
+    R.save_index_bin(idx, tmp_path / "i16.bin", layout="fp16")
+    a = R.load_index_bin(tmp_path / "i8.bin")
+    b = R.load_index_bin(tmp_path / "i16.bin")
+    assert np.array_equal(a.embeds8, idx.embeds8)
+    assert np.array_equal(b.embeds8, idx.embeds8)
+    assert np.array_equal(b.embeds16, idx.embeds16)
+    assert np.array_equal(a.rec_of_col, idx.rec_of_col) and np.array_equal(a.starts, idx.starts)
+    v0, l0 = run(torch_gpu, idx, re, exact, "fp8_rescore")
+    v1, l1 = run(torch_gpu, a, re, exact, "fp8_rescore")
+    assert np.array_equal(l0, l1) and np.array_equal(v0, v1)
