@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (20; human: 5)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (3; human: 1)")
     ap.add_argument("--reads", type=int, default=65_536)
-    ap.add_argument("--fp16-steps", type=int, default=1)
+    ap.add_argument("--fp16-steps", type=int, default=1, help="timed steps of the FP16 path (>= 1)")
     ap.add_argument("--cpu-reads", type=int, default=0, help="0 = size the CPU sample to ~15 s")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cfg1", action="store_true",
@@ -201,6 +201,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
+    if args.fp16_steps < 1:
+        ap.error("--fp16-steps must be at least 1 (the FP16 path is the comparison line)")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(no multi-GPU figure is measured)"}))

@@ -316,8 +316,7 @@ __device__ __forceinline__ void ring_load(const RingSrc &d, uint8_t *sb, int kb)
 // stage), then the barrier that publishes it.  Written out so that the wait
 // stays at that count instead of the vmcnt(0) a fence would add.
 template <class SH>
-__device__ __forceinline__ void ring_wait(int wid) {
-    static_assert(kRing == 4, "the waits below assume two younger stages in flight");
+__device__ __forceinline__ void ring_wait_asm(int wid) {
     if constexpr (SH::BR == 256) {
         asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
     } else if (wid < 4) {
@@ -328,6 +327,18 @@ __device__ __forceinline__ void ring_wait(int wid) {
         static_assert(SH::BR == 64, "ring tiles of 256, 128 or 64 queries");
         asm volatile("s_waitcnt vmcnt(2) lgkmcnt(0)\n\ts_barrier" ::: "memory");
     }
+}
+
+// The sched_barriers keep the machine scheduler from moving any instruction
+// across the wait and barrier: the fragment reads carry alias scopes apart
+// from the loads (below), and the scheduler was seen to sink such reads past
+// a barrier asm, after which their buffer is refilled.
+template <class SH>
+__device__ __forceinline__ void ring_wait(int wid) {
+    static_assert(kRing == 4, "the waits below assume two younger stages in flight");
+    __builtin_amdgcn_sched_barrier(0);
+    ring_wait_asm<SH>(wid);
+    __builtin_amdgcn_sched_barrier(0);
 }
 
 // Waves 4-7 of a ring tile (the second wave on each SIMD) can run one stage
@@ -442,7 +453,9 @@ __device__ __forceinline__ void tile_product_ring(v16f (&acc)[SH::AI][2], const 
     if (s + 2 < nk) ring_stage<ELEM, SH, LAG, false>(acc, f, d, b2, b3, kl(s + 2), wi, wr, lane);
     // every load of the tile (the clamped ones too) has landed and every
     // fragment read is done before the epilogue reuses the ring
+    __builtin_amdgcn_sched_barrier(0);
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
     ring_lag_tail<ELEM, SH, LAG>(acc, f);
 }
 
@@ -498,7 +511,9 @@ __device__ __forceinline__ void tile_product_ring_stream(v16f (&acc)[SH::AI][2],
     ring_stage<ELEM, SH, LAG, false>(acc, f, dn, b1, b2, kn + (has_next ? kBK : 0), wi, wr, lane);
     ring_stage<ELEM, SH, LAG, false, false>(acc, f, dn, b2, b3, 0, wi, wr, lane);
     // every fragment read is done before the epilogue uses buffers 2-3
+    __builtin_amdgcn_sched_barrier(0);
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
     ring_lag_tail<ELEM, SH, LAG>(acc, f);
 }
 
