@@ -39,7 +39,7 @@ Legs beside the headline step:
                     1000 req/s (reads per request log-uniform on [1, 100,000],
                     err in [0, 0.10]) through the continuous-batching server
                     (16,384-row launches, 2 slots, SRPT; low-load mode of
-                    4,096-row whole-request launches while <= 16,384 rows wait):
+                    4,096-row whole-request launches while <= 131,072 rows wait):
                     p50 / p99 / p999, offered vs achieved reads/s, recall
   fasta             the batch as FASTA text, parsed and 2-bit packed on the GPU
   cpu_baseline      the oracle's golden path (FP64 encoder, FP64 dots, top-20)
@@ -192,7 +192,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=400, help="requests per rate")
     ap.add_argument("--serve-rows", type=int, default=16384, help="rows per serving launch")
     ap.add_argument("--serve-streams", type=int, default=2, help="serving launch slots")
-    ap.add_argument("--serve-lowload", default="4096,16384",
+    ap.add_argument("--serve-lowload", default="4096,131072",
                     help="launch_rows,backlog_rows of the server's low-load mode (0,0 = off)")
     ap.add_argument("--serve-policy", choices=["fifo", "srpt"], default="srpt",
                     help="order in which queued rows fill a launch")
