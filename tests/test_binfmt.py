@@ -198,3 +198,24 @@ def test_rejects_bad_files(tmp_path):
     I.save_index_bin(d, bad)
     with pytest.raises(R.RotorMapArgumentError):
         I.load_index_bin(bad)
+
+
+def test_legacy_tag_and_empty_index(tmp_path):
+    """Pre-rename "indexflowreal.fp8.v1" files load (RotorMap/index/load.jl:32-36); an index
+    with no windows round-trips as (rdim, 0) arrays."""
+    d = _tiny()
+    p = tmp_path / "legacy.bin"
+    fields = [("format", "indexflowreal.fp8.v1"), ("source", "/x"), ("k", d.k), ("kstep", d.kstep),
+              ("s", d.s), ("m", d.m), ("c", d.c), ("normalize", 0), ("fp8", True), ("n_frag", d.n_frag),
+              ("embeds8", d.embeds8.T), ("norms", np.ascontiguousarray(d.norms).T),
+              ("heads", list(d.heads)), ("starts", d.starts), ("strand", d.strand)]
+    et = {"embeds8": J.jtype("Float8_E4M3FN"), "norms": J.jtype("Float32"), "heads": J.jtype("String"),
+          "starts": J.jtype("Int64"), "strand": J.jtype("UInt8")}
+    J.jls_dump_namedtuple(p, fields, et)
+    e = I.load_index_bin(p)
+    assert np.array_equal(e.embeds8, d.embeds8) and e.heads == d.heads
+    z = I.IndexData(np.zeros((0, 2048), np.uint8), np.zeros((0, 1), np.float32), [],
+                    np.zeros(0, np.int64), np.zeros(0, np.uint8), 20_000, 2_000, 5, 1, 5, 0, 0, "")
+    I.save_index_bin(z, tmp_path / "empty.bin")
+    ez = I.load_index_bin(tmp_path / "empty.bin")
+    assert ez.embeds8.shape == (0, 2048) and ez.heads == [] and len(ez.starts) == 0
