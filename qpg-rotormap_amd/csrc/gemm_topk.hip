@@ -748,23 +748,48 @@ __device__ __forceinline__ void merge_slots_lanes(float *lv, int32_t *lc, const 
 // empty.  All threads take part, 32 lanes per query (lane = entry), so each
 // wave moves two contiguous list rows per access; the barriers around a merge
 // order a store before any later load of the same list.
+//
+// Stored lists come back by LDS DMA, so a wave has all its loads in flight
+// before one wait (loads through registers would wait on each query's, and
+// the 256-query tile has no registers to spare for more in flight).  One
+// 4-byte load per lane fills two rotated rows, queries 2p and 2p + 1: lane l
+// writes position l & 31 of query q = 2p + (l >> 5), i.e. entry
+// ((l & 31) - q) & 31.  Entries at or past k' are never read (the lane merge
+// and store_lists read j < k'), so those lanes load entry 0 again.  A pair's
+// descriptor starts at query 2p's list and ends after the last in-range
+// query's, so a query past M reads zeros.
 template <int BR, int NT, bool ROT>
 __device__ __forceinline__ void stage_lists(float *lv, int32_t *lc, const GemmArgs &a, int64_t r0,
                                             int split, int KP, ListAt<ROT> at, bool stored,
                                             int tid) {
     static_assert(ROT, "staged lists use the rotated 32-entry rows");
-    const int t = opaque(tid), j = t & 31;
-    for (int q = t >> 5; q < BR; q += NT / 32) {
-        if (j >= KP) continue;
-        float v = -INFINITY;
-        int32_t c = INT_MAX;
-        if (stored && r0 + q < a.M) {
-            const int64_t e = ((r0 + q) * a.S + split) * (int64_t)KP + j;
-            v = a.part_v[e];
-            c = a.part_c[e];
+    if (stored) {
+        constexpr int NW = NT / 64;
+        const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const int lane = tid & 63, half = lane >> 5;
+        const int64_t qs = (int64_t)a.S * KP;  // entries from one query's list to the next's
+        for (int p = wid; p < BR / 2; p += NW) {
+            const int64_t qa = r0 + 2 * p;  // wave-uniform
+            if (qa >= a.M) break;
+            const int q = 2 * p + half, j = ((lane & 31) - q) & 31;
+            const int voff = (int)((half * qs + (j < KP ? j : 0)) * 4);
+            const int nrec = (int)(((qa + 1 < a.M ? qs : 0) + KP) * 4);
+            const int64_t e0 = (qa * a.S + split) * (int64_t)KP;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                __builtin_amdgcn_make_buffer_rsrc((void *)(a.part_v + e0), (short)0, nrec, 0x00020000),
+                (__attribute__((address_space(3))) void *)(lv + p * 64), 4, voff, 0, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                __builtin_amdgcn_make_buffer_rsrc((void *)(a.part_c + e0), (short)0, nrec, 0x00020000),
+                (__attribute__((address_space(3))) void *)(lc + p * 64), 4, voff, 0, 0, 0);
         }
-        lv[at(q, j)] = v;
-        lc[at(q, j)] = c;
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    } else {
+        const int t = opaque(tid), j = t & 31;
+        for (int q = t >> 5; q < BR; q += NT / 32) {
+            if (j >= KP) continue;
+            lv[at(q, j)] = -INFINITY;
+            lc[at(q, j)] = INT_MAX;
+        }
     }
     __syncthreads();
 }
@@ -1127,6 +1152,8 @@ int launch_gemm_topk(const GemmArgs &a0, int elem, hipStream_t st) {
     RM_CHECK(a.R == cdiv(a.M, a.BQ), RM_ERR_ARG, "row tiles %lld != ceil(M / %d)", (long long)a.R,
              a.BQ);
     a.capd = topk_cand_slots(a.KP, a.BQ);
+    // list staging addresses two queries' lists with one 32-bit byte offset
+    RM_CHECK(((int64_t)a.S + 1) * a.KP * 4 < INT32_MAX, RM_ERR_ARG, "%d splits at k' = %d", a.S, a.KP);
     a.bound = nullptr;
     if (uses_ring(a.KP) && a.S > 1 && a.M > 0) {
         // the bounds follow the partials (part_bytes holds M more entries);
