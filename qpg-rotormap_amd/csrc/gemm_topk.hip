@@ -711,21 +711,38 @@ __device__ __forceinline__ void merge_slots_lanes(float *lv, int32_t *lc, const 
             v = lv[at(q, j)];
             c = lc[at(q, j)];
         }
-        for (int i = 0; i < ncmax; ++i) {
-            const int ci = q * CP + min(i, max(nc - 1, 0));
-            const float x = cv[ci];
-            const int32_t xc = cc[ci];
-            const uint64_t m = __ballot(beats(v, c, x, xc));
-            const int p = __popc((uint32_t)(m >> (32 * half)));
-            const float vs = __int_as_float(lane_up(__float_as_int(v)));
-            const int32_t cs = lane_up(c);
-            if (i < nc && p < KP) {
-                if (j == p) {
-                    v = x;
-                    c = xc;
-                } else if (j > p) {
-                    v = vs;
-                    c = cs;
+        // candidates are read G at a time, so a merge step waits on LDS once
+        // per G candidates (reads past a half's last candidate repeat it;
+        // they are not placed)
+        constexpr int G = 4;
+        const int cb = q * CP, cl = max(nc - 1, 0);
+        for (int i0 = 0; i0 < ncmax; i0 += G) {
+            float xs[G];
+            int32_t xcs[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                xs[u] = cv[cb + min(i0 + u, cl)];
+                xcs[u] = cc[cb + min(i0 + u, cl)];
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int i = i0 + u;
+                if (i >= ncmax) break;  // wave-uniform
+                const float x = xs[u];
+                const int32_t xc = xcs[u];
+                // beats(v, c, x, xc) without the short-circuit branch
+                const uint64_t m = __ballot((v > x) | ((v == x) & (c < xc)));
+                const int p = __popc((uint32_t)(m >> (32 * half)));
+                const float vs = __int_as_float(lane_up(__float_as_int(v)));
+                const int32_t cs = lane_up(c);
+                if (i < nc && p < KP) {
+                    if (j == p) {
+                        v = x;
+                        c = xc;
+                    } else if (j > p) {
+                        v = vs;
+                        c = cs;
+                    }
                 }
             }
         }
