@@ -621,6 +621,24 @@ __global__ __launch_bounds__(SH::NT, 1) void gemm_store_kernel(GemmArgs a) {
     }
 }
 
+// max(a, b, c) of accumulators in one v_max3_f32.  fmaxf first quiets each
+// operand it cannot prove canonical (an MFMA result) with v_max_f32 x, x, x,
+// which tripled the per-tile filter's maximum.  The accumulators are never
+// signalling NaNs, and v_max3_f32 returns the larger operand of a quiet NaN
+// and a number, as fmaxf does.
+__device__ __forceinline__ float max3_acc(float a, float b, float c) {
+    float r;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// The median of a, b, c (v_med3_f32), for the same accumulators.
+__device__ __forceinline__ float med3_acc(float a, float b, float c) {
+    float r;
+    asm("v_med3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // An opaque copy of v.  Addresses computed from it stay in the rarely taken
 // paths that use them, instead of being hoisted out of the tile loop and
 // held in registers (or spilled) across the K loop.
@@ -966,9 +984,10 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
                     float m1 = -INFINITY, m2 = -INFINITY;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
+                        // m1 >= m2: the new second largest is the median
                         const float v = acc[ai][bj][i];
-                        m2 = fmaxf(m2, fminf(m1, v));
-                        m1 = fmaxf(m1, v);
+                        m2 = med3_acc(m1, m2, v);
+                        m1 = max3_acc(m1, v, v);
                     }
                     bound = fminf(bound, KP <= 16 ? m1 : m2);
                 }
@@ -1004,7 +1023,7 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
             for (int ai = 0; ai < AI; ++ai)
 #pragma unroll
                 for (int i = 0; i < 16; i += 2)
-                    mx = fmaxf(mx, fmaxf(acc[ai][bj][i], acc[ai][bj][i + 1]));
+                    mx = max3_acc(mx, acc[ai][bj][i], acc[ai][bj][i + 1]);
             if (qv[bj] && mx > tau) {
 #pragma unroll
                 for (int ai = 0; ai < AI; ++ai)
