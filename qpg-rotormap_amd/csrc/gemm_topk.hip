@@ -704,74 +704,79 @@ __device__ __forceinline__ int lane_up(int x) {
     return __builtin_amdgcn_update_dpp(x, x, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
 }
 
+// (value, column) as one signed 64-bit key whose order is beats(): the
+// value's order key above, INT_MAX - column below (columns are >= 0).  The
+// key keeps the value's bits; it orders -0 below +0, which beats() ties, but
+// no score is -0: MFMA sums start from +0, and an exact zero sum rounds to +0.
+__device__ __forceinline__ int64_t merge_key(float v, int32_t c) {
+    return (int64_t)((uint64_t)(uint32_t)order_key(v) << 32 | (uint32_t)(INT_MAX - c));
+}
+__device__ __forceinline__ float merge_key_value(int64_t k) { return order_key_inv((int)(k >> 32)); }
+__device__ __forceinline__ int32_t merge_key_col(int64_t k) { return INT_MAX - (int32_t)(uint32_t)k; }
+
 // The same merge as merge_slots for lists staged as rotated 32-entry rows
-// (k' <= 32), with 32 lanes per query: lane j holds entry j in registers.  A
-// candidate's place p is the number of the half-wave's entries that beat it
-// (a ballot; the 32 lanes stay sorted, so p >= k' only for a candidate that
-// drops out), and lanes j > p take lane j - 1's entry (the shift into lane 32
-// from lane 31 is never taken: p >= 0).  Keys (value, column) are distinct,
-// so the lists equal those of one-by-one insertion.  All NT threads take part,
-// two queries per wave.
+// (k' <= 32), with 32 lanes per query: lane j holds entry j as a merge key.
+// A candidate's place p is the number of the half-wave's 32 keys above it (a
+// ballot), and lanes j > p take lane j - 1's key: a sorted insertion into the
+// half-wave's 32 keys (the shift into lane 32 from lane 31 is never taken:
+// p >= 0).  Lanes k' and up hold the list's tail past k' (or fillers), which
+// is never stored, so no candidate needs a p < k' test.  A half past its last
+// candidate inserts fillers ((-inf, INT_MAX), the key of an empty entry):
+// above no key, so p = the half's real entries, and the lanes from p on hold
+// fillers already.  Keys are distinct, so the lists equal those of one-by-one
+// insertion.  All NT threads take part, two queries per wave.
 template <int BR, int NT>
 __device__ __forceinline__ void merge_slots_lanes(float *lv, int32_t *lc, const float *cv,
                                                   const int32_t *cc, float *tauv, int *cnt,
                                                   int KP, ListAt<true> at, int CP, int capd,
                                                   int tid) {
     const int t = opaque(tid), lane = t & 63, j = lane & 31, half = lane >> 5;
+    const int64_t filler = merge_key(-INFINITY, INT_MAX);
     for (int q0 = (t >> 6) * 2; q0 < BR; q0 += NT / 32) {
         const int q = q0 + half;
         const int nc = min(cnt[q], capd);
         const int ncmax = max(__builtin_amdgcn_readlane(nc, 0), __builtin_amdgcn_readlane(nc, 32));
         if (ncmax == 0) continue;
-        float v = -INFINITY;
-        int32_t c = INT_MAX;
-        if (j < KP) {
-            v = lv[at(q, j)];
-            c = lc[at(q, j)];
-        }
+        int64_t e = filler;
+        if (j < KP) e = merge_key(lv[at(q, j)], lc[at(q, j)]);
         // candidates are read G at a time, so a merge step waits on LDS once
-        // per G candidates (reads past a half's last candidate repeat it;
-        // they are not placed)
+        // per G candidates
         constexpr int G = 4;
         const int cb = q * CP, cl = max(nc - 1, 0);
         for (int i0 = 0; i0 < ncmax; i0 += G) {
-            float xs[G];
-            int32_t xcs[G];
+            int64_t xs[G];
 #pragma unroll
             for (int u = 0; u < G; ++u) {
-                xs[u] = cv[cb + min(i0 + u, cl)];
-                xcs[u] = cc[cb + min(i0 + u, cl)];
+                const int ci = cb + min(i0 + u, cl);
+                const int64_t k = merge_key(cv[ci], cc[ci]);
+                xs[u] = i0 + u < nc ? k : filler;
             }
+            // all G keys are formed here: the compiler would otherwise sink
+            // each read past the early exits into its own step and wait there
+            // (the 256-query tile, at its register limit, ran 1 % slower with
+            // the 8 registers this holds)
+            static_assert(G == 4, "one operand per key");
+            if constexpr (BR < 256)
+                asm volatile("" : "+v"(xs[0]), "+v"(xs[1]), "+v"(xs[2]), "+v"(xs[3]));
 #pragma unroll
             for (int u = 0; u < G; ++u) {
-                const int i = i0 + u;
-                if (i >= ncmax) break;  // wave-uniform
-                const float x = xs[u];
-                const int32_t xc = xcs[u];
-                // beats(v, c, x, xc) without the short-circuit branch
-                const uint64_t m = __ballot((v > x) | ((v == x) & (c < xc)));
+                if (i0 + u >= ncmax) break;  // wave-uniform
+                const int64_t x = xs[u];
+                const uint64_t m = __ballot(e > x);
                 const int p = __popc((uint32_t)(m >> (32 * half)));
-                const float vs = __int_as_float(lane_up(__float_as_int(v)));
-                const int32_t cs = lane_up(c);
-                if (i < nc && p < KP) {
-                    if (j == p) {
-                        v = x;
-                        c = xc;
-                    } else if (j > p) {
-                        v = vs;
-                        c = cs;
-                    }
-                }
+                const int64_t es = (int64_t)((uint64_t)(uint32_t)lane_up((int)(e >> 32)) << 32 |
+                                             (uint32_t)lane_up((int)e));
+                e = j == p ? x : (j > p ? es : e);
             }
         }
         // a query without candidates keeps its list and tau (the first-tile
         // seed may sit above its still empty list)
         if (nc > 0) {
             if (j < KP) {
-                lv[at(q, j)] = v;
-                lc[at(q, j)] = c;
+                lv[at(q, j)] = merge_key_value(e);
+                lc[at(q, j)] = merge_key_col(e);
             }
-            if (j == KP - 1) tauv[q] = v;
+            if (j == KP - 1) tauv[q] = merge_key_value(e);
             if (j == 0) cnt[q] = 0;
         }
     }
