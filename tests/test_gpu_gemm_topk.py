@@ -155,6 +155,35 @@ def test_fused_topk_ties_and_k64(torch_gpu):
                 np.testing.assert_array_equal(v, ev, err_msg=f"k={k} tile={tile} S={S}")
 
 
+def test_fused_topk_zero_score_ties(torch_gpu):
+    """Scores of exactly +0 tie across hundreds of columns: every third index
+    column is all zeros, the other columns score below zero (negative index
+    codes against positive queries) except a few positive ones.  The top-k
+    is the few positive columns, then the zero columns in column order, so
+    the lists fill with equal values and every merge places candidates by
+    the column rule alone."""
+    torch = torch_gpu
+    K, M, N = 256, 200, 3000
+    B = rand_e4m3((N, K), 0.05, 31) | np.uint8(0x80)  # negative or -0 codes
+    B[::3] = 0  # +0 columns
+    B[[5, 700, 2999]] &= np.uint8(0x7F)  # a few positive columns
+    Q = rand_e4m3((M, K), 0.05, 32) & np.uint8(0x7F)  # positive queries
+    assert not np.isin(B, (0x7F, 0xFF)).any() and not np.isin(Q, (0x7F, 0xFF)).any()  # no NaN codes
+    C = gemm_c(torch, Q, B)
+    assert (C[:, ::3] == 0).all() and (np.signbit(C[:, ::3]) == 0).all()
+    for k in (20, 32):
+        ev, el = O.topk_rows(C, k)
+        assert (ev[:, 3:] == 0).all()  # the ties on +0 fill every list past 3 entries
+        eng = R.TopKEngine(B, k=k, mode="fp8")
+        for tile in TILES:
+            for S in (1, 4):
+                v, l = fused(torch, eng, Q, splits=S, tile=tile)
+                msg = f"k={k} tile={tile} S={S}"
+                np.testing.assert_array_equal(l, el, err_msg="locs, " + msg)
+                np.testing.assert_array_equal(v, ev, err_msg="vals, " + msg)
+                assert (np.signbit(v) == np.signbit(ev)).all(), "sign of zero, " + msg
+
+
 def test_fused_topk_fp16_bitexact(torch_gpu):
     torch = torch_gpu
     rng = np.random.default_rng(21)
