@@ -704,6 +704,22 @@ __device__ __forceinline__ int lane_up(int x) {
     return __builtin_amdgcn_update_dpp(x, x, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
 }
 
+// OR of x over the wave's 64 lanes, returned wave-uniform (a row_shr scan
+// within each row of 16 lanes, then row_bcast:15 and row_bcast:31 carry the
+// rows' ORs up to lane 63).  Every lane must be active.
+__device__ __forceinline__ uint32_t wave_or32(uint32_t x) {
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0x112 /* row_shr:2 */, 0xf, 0xf, false);
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0x114 /* row_shr:4 */, 0xf, 0xf, false);
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0x118 /* row_shr:8 */, 0xf, 0xf, false);
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0x142 /* row_bcast:15 */, 0xa, 0xf, false);
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0x143 /* row_bcast:31 */, 0xc, 0xf, false);
+    return __builtin_amdgcn_readlane(x, 63);
+}
+__device__ __forceinline__ uint64_t wave_or64(uint64_t x) {
+    return (uint64_t)wave_or32((uint32_t)(x >> 32)) << 32 | wave_or32((uint32_t)x);
+}
+
 // (value, column) as one signed 64-bit key whose order is beats(): the
 // value's order key above, INT_MAX - column below (columns are >= 0).  The
 // key keeps the value's bits; it orders -0 below +0, which beats() ties, but
@@ -1040,23 +1056,26 @@ __device__ __forceinline__ void topk_item(const GemmArgs &a, int64_t rt, int spl
         // One vote per tile: lanes append without a prior vote (a slot is
         // reserved with an atomic, and merges only run after the vote below)
         for (;;) {
-            // --- append flagged values to the query's slots; a 16-value
-            // group with no flag in any lane of the wave is skipped
+            // --- append flagged values to the query's slots.  A position
+            // (ai, i) no lane of the wave flags is skipped on the wave-uniform
+            // OR of the flags, with a scalar test instead of a per-lane one
 #pragma unroll
             for (int bj = 0; bj < BB; ++bj) {
-                if (!pend[bj]) continue;
+                if (!__ballot(pend[bj] != 0)) continue;  // wave-uniform
+                const uint64_t any_lane = wave_or64(pend[bj]);
                 const int q = ql[bj];
                 // one atomic reserves a slot per flagged value; the first
                 // capd - slot of them (in (ai, i) order) are written, the
                 // rest stay pending for the retry after the merge
-                int slot = atomicAdd(&cnt[q], __popcll(pend[bj]));
-                if (slot >= capd) continue;
+                int slot = capd;
+                if (pend[bj]) slot = atomicAdd(&cnt[q], __popcll(pend[bj]));
 #pragma unroll
                 for (int ai = 0; ai < AI; ++ai) {
-                    if (!((pend[bj] >> (ai * 16)) & 0xffffull)) continue;
+                    if (!((any_lane >> (ai * 16)) & 0xffffull)) continue;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
                         const uint64_t bit = 1ull << (ai * 16 + i);
+                        if (!(any_lane & bit)) continue;  // wave-uniform
                         if ((pend[bj] & bit) && slot < capd) {
                             cv[q * CP + slot] = acc[ai][bj][i];
                             cc[q * CP + slot] = colbase + RC(ai, i);
