@@ -212,7 +212,7 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words,
  * tiles grouped per XCD qg) per (mode, batch-size bucket ceil(log2 M)); the
  * reference's fixed chunk width w and rows_cap (RotorMap/search/
  * engine_fp8.jl:309-337) are what it replaces.  Without a tuned entry a
- * batch takes 64 rows up to 64, 128 up to 256, else 256, and the
+ * batch takes 64 rows up to 64, 128 up to 128, else 256, and the
  * built-in split rule, stepping the tile down while a split would sweep
  * fewer than 32 column tiles of the index; lists longer than 32 (kp > 32)
  * always run 128-row tiles.  Results never depend on the plan.

@@ -105,13 +105,15 @@ static int bucket_of(int64_t M) {
 }
 
 // Query rows per tile without a tuned plan: the smallest ring tile that
-// holds up to 128 rows (a small batch wastes less MFMA work on padded query
-// rows), the 128-row tile up to 256 rows (6 % faster than the 256-row one
-// there, measured at human scale), then 256; lists longer than 32 have only
-// the 128 tile.  plan_for may step it down for short splits.
+// holds the batch up to 128 rows (a small batch wastes less MFMA work on
+// padded query rows), then 256; lists longer than 32 have only the 128
+// tile.  Above 128 rows the 256-row tile with one row tile and 256 splits
+// beats two 128-row tiles (human scale: 1.94 / 2.07 / 2.29 ms at 160 / 192
+// / 256 reads against 2.69 / 2.47 / 2.41 ms).  plan_for may step it down
+// for short splits.
 static int default_tile(int64_t M, int kp) {
     if (!ring_list(kp)) return 128;
-    return M <= 64 ? 64 : M <= 256 ? 128 : 256;
+    return M <= 64 ? 64 : M <= 128 ? 128 : 256;
 }
 
 // Tile plan for a batch: (query rows per tile, splits, query tiles per XCD

@@ -424,7 +424,7 @@ def test_index_past_2_32_elements(torch_gpu):
 
 def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
     """With no tuned plan, a batch takes the smallest ring tile that holds it
-    (64 rows up to 64, 128 up to 256, else 256) and the smallest split count
+    (64 rows up to 64, 128 up to 128, else 256) and the smallest split count
     (from 'one item per CU' up, doubling) for which the query tiles worked on
     at once, min(R, CUs / S) tiles of that many rows, fit in 128 MB: a
     65,536-read e4m3 batch (2 KB rows, 128 MB) stays unsplit on 256 CUs; the
@@ -452,7 +452,7 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
             return S
 
         def expect(M, row_bytes, kp=20):
-            bq = 128 if kp > 32 else (64 if M <= 64 else 128 if M <= 256 else 256)
+            bq = 128 if kp > 32 else (64 if M <= 64 else 128 if M <= 128 else 256)
             S = splits(M, bq, row_bytes)
             while kp <= 32 and bq > 64 and -(-tiles // S) < 32:
                 bq //= 2
