@@ -215,11 +215,23 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words,
  * batch takes 64 rows up to 64, 128 up to 128, else 256, and the
  * built-in split rule, stepping the tile down while a split would sweep
  * fewer than 32 column tiles of the index; lists longer than 32 (kp > 32)
- * always run 128-row tiles.  Results never depend on the plan.
- * rm_plan_get: the plan a batch of M rows gets; kp is the list length the
+ * always run 128-row tiles.  With splits and at least CUs / 8 query tiles
+ * the query tiles are grouped CUs / 8 / S per XCD.  Results never depend on
+ * the plan.
+ * rm_plan_get: the plan one launch of M rows gets; kp is the list length the
  * fused kernel keeps (kprime for RM_MODE_FP8_RESCORE, else ktop). */
 int rm_plan_get(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int32_t *tile_rows,
                 int32_t *splits, int32_t *qgroup);
+/* Rows per fused launch that rm_topk_batch and rm_map_batch use for a batch
+ * of M rows (0: one launch).  Without a tuned plan for M's bucket, a batch
+ * above 64 x CUs rows on a large index (each of the launches' splits
+ * sweeping at least 1,024 column tiles) runs as equal launches of at most
+ * that many rows, each with rm_plan_get's plan for that many rows: one wave
+ * of workgroups with its query tiles resident in the XCDs' L2.  Uneven
+ * batches (rm_map_batches) always run as one launch.  No counterpart in the
+ * reference, whose engine cuts batches at its own rows_cap
+ * (RotorMap/search/engine_fp8.jl:309-337). */
+int rm_launch_rows(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int64_t *rows);
 /* Pin the plan for batches in M's bucket: tile_rows 0 (the default tile),
  * 64, 128 or 256; splits 0 (the default rule) .. 256; qgroup 0 (all) or the
  * query tiles grouped per XCD.  tile_rows = splits = 0 clears the entry. */

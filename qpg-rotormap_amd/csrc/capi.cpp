@@ -145,8 +145,36 @@ Plan plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
         p.bq /= 2;
         p.S = default_splits(ctx, M, idx->N, p.bq, rb);
     }
-    p.qg = 0;
+    // With splits and at least an XCD's worth of query tiles, group them so
+    // that each XCD's resident workgroups (CUs / 8) hold whole groups of
+    // CUs / 8 / S query tiles x S splits: those query tiles stay in the XCD's
+    // L2 instead of being re-read from the MALL for every column tile.  One
+    // group over all query tiles gave an XCD 32 tiles of one split (human
+    // scale: 8,192 rows 45.6 ms against 41.2 ms grouped, 16,384 rows 87.9
+    // against 81.0, 32,768 rows 171.5 against 165.3).
+    const int64_t xcd_items = std::max(1, ctx->num_cus / 8);
+    p.qg = (p.S > 1 && cdiv(M, p.bq) >= xcd_items) ? (int)std::max<int64_t>(1, xcd_items / p.S) : 0;
     return p;
+}
+
+// Rows per launch for a batch without a tuned plan, or 0 for one launch.
+// Above 64 x CUs rows (16,384 on 256 CUs) a batch runs as equal launches of
+// at most that many rows: each is one wave of CUs / 4 query tiles x 4 splits
+// with L2-resident query tiles (grouped as above), and every launch starts
+// all workgroups together.  One launch of a larger batch gives each
+// workgroup several items, whose drifting starts mix query groups on an XCD
+// (human scale, alternated: a 65,536-read step 319.0 ms as four launches
+// against 334.9 ms as one, with the same rows).  A split must still sweep a
+// long column range, or the launches add list warm-up (kChunkSplitTiles).
+constexpr int64_t kChunkSplitTiles = 1024;
+int64_t launch_rows(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp) {
+    const int64_t L = 64LL * ctx->num_cus;
+    if (!ring_list(kp) || M <= L) return 0;
+    if (ctx->plan.count({mode, bucket_of(M)})) return 0;  // a tuned plan runs as one launch
+    const int64_t n = cdiv(M, L), rows = std::min<int64_t>(L, cdiv(cdiv(M, n), 256) * 256);
+    const Plan pc = plan_for(ctx, idx, rows, mode, kp);
+    if (cdiv(cdiv(idx->N, kBI), pc.S) < kChunkSplitTiles) return 0;
+    return rows;
 }
 
 static int clamp_splits(const rm_index *idx, int S) {
@@ -177,6 +205,28 @@ int topk_core(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes
     if ((rc = ws_get(ctx, "part_c", pbytes, &pc))) return rc;
     return topk_core_bufs(ctx, idx, Q, ldq_bytes, elem, Q32, ldq32, M, ktop, kprime, mode, vals,
                           locs, ldo, pl, (float *)pv, (int32_t *)pc, pbytes, st);
+}
+
+// A batch under the built-in launch rule: launch_rows, then plan_for per
+// launch.  Rows are independent, so launches of consecutive row ranges give
+// the rows of one launch.
+int topk_batched(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
+                 const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
+                 float *vals, int32_t *locs, int64_t ldo, hipStream_t st) {
+    const int kp = mode == RM_MODE_FP8_RESCORE ? kprime : ktop;
+    const int64_t rows = launch_rows(ctx, idx, M, mode, kp);
+    if (rows == 0)
+        return topk_core(ctx, idx, Q, ldq_bytes, elem, Q32, ldq32, M, ktop, kprime, mode, vals, locs,
+                         ldo, plan_for(ctx, idx, M, mode, kp), st);
+    const Plan pl = plan_for(ctx, idx, rows, mode, kp);
+    for (int64_t r0 = 0; r0 < M; r0 += rows) {
+        const int64_t Mc = std::min(rows, M - r0);
+        int rc = topk_core(ctx, idx, (const uint8_t *)Q + r0 * ldq_bytes, ldq_bytes, elem,
+                           Q32 ? Q32 + r0 * ldq32 : nullptr, ldq32, Mc, ktop, kprime, mode, vals + r0,
+                           locs + r0, ldo, pl, st);
+        if (rc) return rc;
+    }
+    return RM_OK;
 }
 
 int topk_core_bufs(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
@@ -622,9 +672,8 @@ int rm_topk_batch(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq, 
     if (mode == RM_MODE_FP8_RESCORE) RM_CHECK(Q32 && ldq32 >= idx->rdim, RM_ERR_ARG, "Q32 required");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     RM_HIP(hipSetDevice(ctx->device));
-    const Plan pl = plan_for(ctx, idx, M, mode, mode == RM_MODE_FP8_RESCORE ? kprime : ktop);
-    return topk_core(ctx, idx, Q, ldq_b, elem, Q32, ldq32, M, ktop, kprime, mode, vals, locs, ldo,
-                     pl, pick_stream(ctx, stream));
+    return topk_batched(ctx, idx, Q, ldq_b, elem, Q32, ldq32, M, ktop, kprime, mode, vals, locs, ldo,
+                        pick_stream(ctx, stream));
 }
 
 // encode n fragments straight into the query layouts the GEMM reads
@@ -660,13 +709,12 @@ static int encode_queries(rm_ctx *ctx, const uint32_t *words, int64_t n, int k, 
     return encode_into(ctx, words, n, k, s, m, c, mode, rdim, norms, *q, *q32, st);
 }
 
-}  // namespace rm
-
-extern "C" {
-
-int rm_map_batch(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_t n, int k,
-                 int s, int m, int c, int ktop, int kprime, int mode, float *vals, int32_t *locs,
-                 int64_t ldo, float *norms, void *stream) {
+// encode -> fused GEMM + top-k -> merge of n fragments; one_launch keeps the
+// whole batch in one fused launch (the uneven-batch path), else the
+// built-in launch rule applies (topk_batched)
+static int map_batch_impl(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_t n,
+                          int k, int s, int m, int c, int ktop, int kprime, int mode, float *vals,
+                          int32_t *locs, int64_t ldo, float *norms, void *stream, bool one_launch) {
     RM_CHECK(ctx && idx && vals && locs, RM_ERR_ARG, "NULL argument");
     int rc;
     if ((rc = check_encoder(k, s, m, c, 0))) return rc;
@@ -684,9 +732,23 @@ int rm_map_batch(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_
     float *q32;
     if ((rc = encode_queries(ctx, words, n, k, s, m, c, mode, rdim, norms, &q, &q32, st))) return rc;
     const int elem = mode == RM_MODE_FP16 ? 1 : 0;
-    const Plan pl = plan_for(ctx, idx, n, mode, mode == RM_MODE_FP8_RESCORE ? kprime : ktop);
-    return topk_core(ctx, idx, q, rdim * (elem ? 2 : 1), elem, q32, rdim, n, ktop, kprime, mode,
-                     vals, locs, ldo, pl, st);
+    if (one_launch)
+        return topk_core(ctx, idx, q, rdim * (elem ? 2 : 1), elem, q32, rdim, n, ktop, kprime, mode,
+                         vals, locs, ldo,
+                         plan_for(ctx, idx, n, mode, mode == RM_MODE_FP8_RESCORE ? kprime : ktop), st);
+    return topk_batched(ctx, idx, q, rdim * (elem ? 2 : 1), elem, q32, rdim, n, ktop, kprime, mode,
+                        vals, locs, ldo, st);
+}
+
+}  // namespace rm
+
+extern "C" {
+
+int rm_map_batch(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_t n, int k,
+                 int s, int m, int c, int ktop, int kprime, int mode, float *vals, int32_t *locs,
+                 int64_t ldo, float *norms, void *stream) {
+    return map_batch_impl(ctx, idx, words, n, k, s, m, c, ktop, kprime, mode, vals, locs, ldo, norms,
+                          stream, false);
 }
 
 int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, const int64_t *offs,
@@ -701,8 +763,8 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, cons
     // rows are independent: the concatenation is one launch whose (row tile,
     // split) work list spans every batch -- no per-batch launch, no per-batch
     // tail; the split count follows the total row count
-    return rm_map_batch(ctx, idx, words, offs[nb], k, s, m, c, ktop, kprime, mode, vals, locs, ldo,
-                        nullptr, stream);
+    return map_batch_impl(ctx, idx, words, offs[nb], k, s, m, c, ktop, kprime, mode, vals, locs, ldo,
+                          nullptr, stream, true);
 }
 
 int rm_plan_get(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int32_t *tile_rows,
@@ -714,6 +776,14 @@ int rm_plan_get(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, i
     if (tile_rows) *tile_rows = p.bq;
     if (splits) *splits = p.S;
     if (qgroup) *qgroup = p.qg;
+    return RM_OK;
+}
+
+int rm_launch_rows(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int64_t *rows) {
+    RM_CHECK(ctx && idx && rows && M >= 1, RM_ERR_ARG, "NULL argument or M < 1");
+    RM_CHECK(kp >= 1 && kp <= kMaxKP, RM_ERR_ARG, "kp must be 1..%d", kMaxKP);
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    *rows = launch_rows(ctx, idx, M, mode, kp);
     return RM_OK;
 }
 
@@ -794,6 +864,29 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
                                                (int64_t)(4e12 / (2.0 * M * rdim)));
         sub.N = std::min<int64_t>(idx->N, ncap);
         const int64_t stiles = cdiv(sub.N, kBI);
+        // a batch the built-in rule splits into launches (on the whole index)
+        // keeps that rule: its launches take the tuned plan of their own
+        // bucket, so no single-launch plan is stored for it
+        ctx->plan.erase({mode, bucket_of(M)});
+        if (const int64_t rows = launch_rows(ctx, idx, M, mode, kp)) {
+            const Plan pc = plan_for(ctx, idx, rows, mode, kp);
+            auto run = [&]() {
+                return topk_batched(ctx, &sub, q, rdim * esz, elem, (float *)q32, rdim, M, ktop,
+                                    kprime, mode, (float *)vo, (int32_t *)lo, M, st);
+            };
+            if ((rc = run())) return rc;  // warm-up
+            RM_HIP(hipEventRecord(ev.e0, st));
+            if ((rc = run())) return rc;
+            RM_HIP(hipEventRecord(ev.e1, st));
+            RM_HIP(hipEventSynchronize(ev.e1));
+            float ms = 0;
+            RM_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
+            if (best_tile) best_tile[b] = pc.bq;
+            if (best_splits) best_splits[b] = pc.S;
+            if (best_qgroup) best_qgroup[b] = pc.qg;
+            if (best_ms) best_ms[b] = ms;
+            continue;
+        }
         // tiles: every ring tile up to the first that holds the batch; splits
         // from a quarter of one wave of workgroups to four waves
         std::vector<Plan> cands;

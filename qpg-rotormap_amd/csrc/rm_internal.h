@@ -275,4 +275,10 @@ int check_query(const rm_index *idx, int ktop, int kprime, int mode);
 // the tile plan of a batch of M rows (kp: the list length the fused kernel
 // keeps, kprime for RM_MODE_FP8_RESCORE and ktop otherwise)
 Plan plan_for(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp);
+// rows per fused launch under the built-in rule for a batch of M rows (0:
+// one launch), and the query path over such launches
+int64_t launch_rows(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp);
+int topk_batched(rm_ctx *ctx, const rm_index *idx, const void *Q, int64_t ldq_bytes, int elem,
+                 const float *Q32, int64_t ldq32, int64_t M, int ktop, int kprime, int mode,
+                 float *vals, int32_t *locs, int64_t ldo, hipStream_t st);
 }  // namespace rm

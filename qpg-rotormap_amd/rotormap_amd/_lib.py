@@ -67,6 +67,7 @@ SIGNATURES = {
                              vp]),
     "rm_plan_get": (i32, [vp, vp, i64, i32, i32, vp, vp, vp]),
     "rm_plan_set": (i32, [vp, i64, i32, i32, i32, i32]),
+    "rm_launch_rows": (i32, [vp, vp, i64, i32, i32, vp]),
     "rm_autotune": (i32, [vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
     "rm_set_persistent": (i32, [vp, i32]),
     "rm_prof_enable": (i32, [vp, i32]),

@@ -166,8 +166,17 @@ class TopKEngine:
                                     L.ptr(qg), L.ptr(ms)))
         return tile, best, qg, ms
 
+    def launch_rows(self, M):
+        """Rows per fused launch for a batch of M rows (0: one launch); each
+        launch runs with plan(launch rows)."""
+        r = C.c_int64()
+        kp = self.kprime if self.mode == L.RM_MODE_FP8_RESCORE else self.k
+        L.check(L.lib().rm_launch_rows(L.ctx(self.device), self.index.handle, int(M), self.mode, kp,
+                                       C.byref(r)))
+        return r.value
+
     def plan(self, M):
-        """(tile rows, splits, query groups) a batch of M rows runs with."""
+        """(tile rows, splits, query groups) one launch of M rows runs with."""
         t, s, g = C.c_int32(), C.c_int32(), C.c_int32()
         kp = self.kprime if self.mode == L.RM_MODE_FP8_RESCORE else self.k
         L.check(L.lib().rm_plan_get(L.ctx(self.device), self.index.handle, int(M), self.mode, kp,

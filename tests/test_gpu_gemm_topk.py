@@ -372,6 +372,48 @@ def test_rowtopk_merge_segments_and_ties(torch_gpu, M, k):
     np.testing.assert_array_equal(Dv.cpu().numpy(), ev)
 
 
+def test_batch_cut_into_launches_bitexact(torch_gpu):
+    """A batch above 64 x CUs rows on an index long enough for the launches'
+    splits (4,297 column tiles, 4 splits of 1,075) runs as equal launches of
+    at most that many rows.  Its rows equal one launch of the whole batch
+    (a pinned plan, which turns the cut off) and separate calls on the
+    launches' row ranges, bit for bit, in plain FP8 and with rescoring."""
+    torch = torch_gpu
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    K, N, k, kp = 64, 1_100_000, 20, 32
+    M = 3 * 64 * cus - 5 * 256 + 13  # three launches of 63 query tiles (4 splits on 256 CUs)
+    B = rand_e4m3((N, K), 0.05, 41)
+    Q = rand_e4m3((M, K), 0.05, 42)
+    rng = np.random.default_rng(43)
+    Q32 = (O.e4m3_table()[Q] + rng.standard_normal((M, K)) * 1e-3).astype(np.float32)
+    dq, d32 = torch.from_numpy(Q).cuda(), torch.from_numpy(Q32).cuda()
+    for mode in ("fp8", "fp8_rescore"):
+        eng = R.TopKEngine(B, k=k, kprime=kp, mode=mode)
+        rows = eng.launch_rows(M)
+        assert rows > 0 and rows % 256 == 0 and rows <= 64 * cus and -(-M // rows) == 3, rows
+        q32 = d32 if mode == "fp8_rescore" else None
+        v, l = eng.batch_gemm_topk(dq, q32)
+        torch.cuda.synchronize()
+        v, l = v.cpu().numpy(), l.cpu().numpy()
+        # one launch: a pinned plan for M's bucket (the launches' own plan)
+        t_, s_, g_ = eng.plan(rows)
+        eng.set_plan(M, t_, s_, g_)
+        assert eng.launch_rows(M) == 0
+        v1, l1 = eng.batch_gemm_topk(dq, q32)
+        torch.cuda.synchronize()
+        eng.set_plan(M, 0, 0, 0)
+        np.testing.assert_array_equal(l1.cpu().numpy(), l, err_msg=mode)
+        np.testing.assert_array_equal(v1.cpu().numpy(), v, err_msg=mode)
+        # the launches' row ranges as separate calls
+        for r0 in range(0, M, rows):
+            r1 = min(M, r0 + rows)
+            vc, lc = eng.batch_gemm_topk(dq[r0:r1], q32[r0:r1] if q32 is not None else None)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(lc.cpu().numpy(), l[r0:r1], err_msg=f"{mode} rows {r0}")
+            np.testing.assert_array_equal(vc.cpu().numpy(), v[r0:r1], err_msg=f"{mode} rows {r0}")
+        eng.index.close()
+
+
 def test_persistent_grid_bitexact(torch_gpu):
     """One workgroup per CU walking the (row tile, split) items and one
     workgroup per item give identical top-k, also when each workgroup runs
@@ -428,10 +470,11 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
     (from 'one item per CU' up, doubling) for which the query tiles worked on
     at once, min(R, CUs / S) tiles of that many rows, fit in 128 MB: a
     65,536-read e4m3 batch (2 KB rows, 128 MB) stays unsplit on 256 CUs; the
-    same batch in fp16 (4 KB rows, 256 MB) takes two splits; a 4,096-read
-    batch takes CUs / 16.  While a split would sweep fewer than 32 of the
-    index's column tiles the tile halves (down to 64 rows): on this
-    512-tile index 100 and 200 reads run 64-row tiles."""
+    same batch in fp16 (4 KB rows, 256 MB) takes two splits, with its query
+    tiles grouped CUs / 8 / 2 per XCD; a 4,096-read batch takes CUs / 16.
+    While a split would sweep fewer than 32 of the index's column tiles the
+    tile halves (down to 64 rows): on this 512-tile index 100 and 200 reads
+    run 64-row tiles.  The index is too short for launches to be cut."""
     torch = torch_gpu
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     idx = R.ResidentIndex(np.zeros((512 * 256, 2048), np.uint8), fp16=True)
@@ -457,7 +500,10 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
             while kp <= 32 and bq > 64 and -(-tiles // S) < 32:
                 bq //= 2
                 S = splits(M, bq, row_bytes)
-            return bq, S, 0
+            # query tiles grouped CUs / 8 / S per XCD once there are CUs / 8 of them
+            xcd = max(1, cus // 8)
+            qg = max(1, xcd // S) if S > 1 and -(-M // bq) >= xcd else 0
+            return bq, S, qg
 
         def plan(M, mode, kp=20):
             t, s_, g = L.C.c_int32(), L.C.c_int32(), L.C.c_int32()
@@ -468,7 +514,9 @@ def test_default_plan_keeps_query_tiles_in_mall(torch_gpu):
         p8, p16 = plan(65_536, L.RM_MODE_FP8), plan(65_536, L.RM_MODE_FP16)
         assert (p8, p16) == (expect(65_536, 2048), expect(65_536, 4096))
         if cus == 256:
-            assert (p8[1], p16[1]) == (1, 2)
+            assert (p8[1], p16[1]) == (1, 2) and p16[2] == 16
+        eng = R.TopKEngine(idx, k=20, mode="fp8")
+        assert eng.launch_rows(65_536) == 0  # 128 column tiles per split: one launch
         for M in (4_096, 200, 100, 30):
             assert plan(M, L.RM_MODE_FP8) == expect(M, 2048), M
         if cus == 256:
