@@ -16,7 +16,10 @@ Workloads (synthetic; one step = one batch of reads resident in HBM):
   index             k = 20,000, kstep = 2,000, s5m1c5 (rdim 2048), fwd + rc
                     windows; the e4m3 index and the classic fp16 rows come
                     from one encoder pass
-  step              encode -> fused e4m3 GEMM + top-32 -> FP32 rescore -> top-20
+  step              encode -> fused e4m3 GEMM + top-32 -> FP32 rescore -> top-20;
+                    the library's built-in launch rule runs a 65,536-read batch
+                    on the human-scale index as four launches of 16,384 reads
+                    (rm_launch_rows), so `roofline` is per launch
 Legs beside the headline step:
   modes             plain FP8 (top-20 of the e4m3 GEMM), FP16 over the
                     verbatim fp16 index (the reference-accuracy pipeline) and
@@ -360,7 +363,9 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP8_TFLOPS, 4),
                      "traffic": traffic_from_profile("gemm_topk_fp8_%s_pmc.json" % args.workload),
                      "kernel": "gemm_topk_kernel<e4m3, 256-query tile>", "kernel_ms": round(k_avg_ms, 3),
-                     "flop_per_launch": flop8 / max(1, nl8)},
+                     "flop_per_launch": flop8 / max(1, nl8),
+                     "reads_per_launch": eng8.launch_rows(M) or M,
+                     "launches_per_step": round(nl8 / steps, 3)},
         "roofline_fp16": {"bound": "mfma", "achieved": round(a16, 2), "peak": PEAK_FP16_TFLOPS,
                           "unit": "TFLOP/s", "frac": round(a16 / PEAK_FP16_TFLOPS, 4),
                           "kernel_ms": round(kms16 / max(1, nl16), 3)},
