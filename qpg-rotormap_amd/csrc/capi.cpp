@@ -884,7 +884,7 @@ int rm_autotune(rm_ctx *ctx, const rm_index *idx, const int64_t *bucket_rows, in
             if (best_tile) best_tile[b] = pc.bq;
             if (best_splits) best_splits[b] = pc.S;
             if (best_qgroup) best_qgroup[b] = pc.qg;
-            if (best_ms) best_ms[b] = ms;
+            if (best_ms) best_ms[b] = ms / (float)cdiv(M, rows);  // per launch
             continue;
         }
         // tiles: every ring tile up to the first that holds the batch; splits

@@ -411,6 +411,15 @@ def test_batch_cut_into_launches_bitexact(torch_gpu):
             torch.cuda.synchronize()
             np.testing.assert_array_equal(lc.cpu().numpy(), l[r0:r1], err_msg=f"{mode} rows {r0}")
             np.testing.assert_array_equal(vc.cpu().numpy(), v[r0:r1], err_msg=f"{mode} rows {r0}")
+        # the autotuner keeps the cut for this bucket: it reports the launches'
+        # plan and stores none, and the rows stay the same
+        tile, best, qg, ms = eng.autotune([M])
+        assert eng.launch_rows(M) == rows
+        assert (int(tile[0]), int(best[0]), int(qg[0])) == eng.plan(rows) and ms[0] > 0
+        v2, l2 = eng.batch_gemm_topk(dq, q32)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(l2.cpu().numpy(), l, err_msg=mode + " after autotune")
+        np.testing.assert_array_equal(v2.cpu().numpy(), v, err_msg=mode + " after autotune")
         eng.index.close()
 
 
