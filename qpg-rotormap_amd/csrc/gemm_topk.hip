@@ -351,12 +351,16 @@ __device__ __forceinline__ void ring_wait(int wid) {
 // same barriers and buffer lifetimes: every read of a stage still completes
 // before the barrier that precedes the refill of its buffer.  Off by
 // default: alternated runs on one MI355X measured it neutral at the
-// human-scale index (332.6 / 332.6 ms against 330.2 / 334.8 ms per
-// 65,536-read launch) and 2.7 % slower at 100 Mbp (13.11 against 12.75 ms).
+// human-scale index as one 65,536-read launch (332.6 / 332.6 ms against
+// 330.2 / 334.8 ms), 2.7 % slower at 100 Mbp (13.11 against 12.75 ms), and,
+// with L2-resident query tiles, 16,384-read launches at 80.0 / 80.2 ms
+// against 77.9 / 77.8 ms (100 Mbp 12.52 against 12.11 ms).
 #ifndef RM_LAG
 #define RM_LAG 0
 #endif
-// RM_SETPRIO: raise the wave priority while it issues a block of MFMAs
+// RM_SETPRIO: raise the wave priority while it issues a block of MFMAs.
+// Off: 16,384-read launches at human scale 78.8 / 79.1 ms against 77.9 /
+// 77.8 ms, 100 Mbp 12.30 against 12.11 ms (alternated, one MI355X).
 #ifndef RM_SETPRIO
 #define RM_SETPRIO 0
 #endif
