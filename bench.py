@@ -36,8 +36,10 @@ Legs beside the headline step:
                     65,536-column C chunks (fp16 or fp32) + rm_rowtopk_merge, chunk
                     by chunk over the index, against the fused launch
   uneven (human)    the first 512 of 4,096 batch sizes log-uniform in
-                    [1, 131,072] (err in [0, 0.10]): one launch per batch vs
-                    one persistent launch, after the autotuner
+                    [1, 131,072] (err in [0, 0.10]): one call per batch vs
+                    one rm_map_batches call over the concatenation (one work
+                    list, cut into launches as one batch of that many rows
+                    would be), after the autotuner
   serving           Poisson traces of 400 requests at 10 / 20 / 50 / 200 /
                     1000 req/s (reads per request log-uniform on [1, 100,000],
                     err in [0, 0.10]) through the continuous-batching server
@@ -607,7 +609,7 @@ def main():
                                model=1 if human else 0)
         log("100000 reads sampled for the uneven and serving legs (err in [0, 0.10])")
 
-    # ---- uneven batches: one launch per batch vs one launch for all ---------
+    # ---- uneven batches: one call per batch vs one call for all -------------
     if human and not args.no_uneven:
         urng = np.random.default_rng(404)
         sizes = np.floor(np.exp(urng.uniform(0, np.log(131_073), 4096))).astype(np.int64)
@@ -639,20 +641,22 @@ def main():
         t_one = time.perf_counter() - t
         ru = recall(lu.cpu().numpy(), spool.recs[rows_u], spool.starts[rows_u])
         fl_u = total * flop_read
+        lr_u = engu.launch_rows(total)
         result["uneven"] = {
             "batches": int(len(sizes)), "reads": total, "sizes": "first %d of 4096 log-uniform "
             "in [1, 131072] (seed 404)" % len(sizes), "err": "uniform [0, 0.10], sub/ins/del",
             "one_launch_per_batch_reads_s": rnd(total / t_each, 1),
-            "single_persistent_launch_reads_s": rnd(total / t_one, 1),
+            "one_call_for_all_reads_s": rnd(total / t_one, 1),
+            "one_call_launches": -(-total // lr_u) if lr_u else 1,
             "roofline": {"bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP8_TFLOPS,
                          "one_launch_per_batch_frac": rnd(fl_u / t_each / 1e12 / PEAK_FP8_TFLOPS, 4),
-                         "single_launch_frac": rnd(fl_u / t_one / 1e12 / PEAK_FP8_TFLOPS, 4)},
+                         "one_call_for_all_frac": rnd(fl_u / t_one / 1e12 / PEAK_FP8_TFLOPS, 4)},
             "autotune_s": rnd(t_tune, 2),
             "autotuned_plans_tile_splits_qgroup": {int(b): [int(t_), int(s_), int(g)]
                                                    for b, t_, s_, g in zip(buckets, tile, best, qg)},
             "recall_top20": rnd(ru, 5)}
         log(f"uneven: {len(sizes)} batches, {total} reads: per-batch {total / t_each:.0f} reads/s, "
-            f"one launch {total / t_one:.0f} reads/s, recall {ru:.5f}")
+            f"one call {total / t_one:.0f} reads/s, recall {ru:.5f}")
         del engu, uw, vu, lu
         torch.cuda.empty_cache()
 

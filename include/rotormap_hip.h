@@ -196,13 +196,14 @@ int rm_map_batch_noalloc(rm_ctx *ctx, const rm_index *idx, const uint32_t *words
                          int k, int s, int m, int c, int ktop, int kprime, int mode, float *vals,
                          int32_t *locs, int64_t ldo, float *norms, void *stream);
 
-/* Uneven batches in ONE launch: batch b holds rows [offs[b], offs[b+1]) of
+/* Uneven batches in one call: batch b holds rows [offs[b], offs[b+1]) of
  * the concatenated words (host offs, nb+1 entries); results land at the same
  * rows of vals/locs.  Rows are independent, so the batches are encoded
  * together and the persistent kernel walks one work list of (row tile,
  * column split) items across all of them, with the tile shape and split
- * count the context's tile plan gives the total row count (rm_autotune).
- * Each row's result equals that of rm_map_batch on its batch alone. */
+ * count the context's tile plan gives the total row count (rm_autotune);
+ * a large total runs as the launches rm_launch_rows gives it.  Each row's
+ * result equals that of rm_map_batch on its batch alone. */
 int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words,
                    const int64_t *offs, int nb, int k, int s, int m, int c, int ktop, int kprime,
                    int mode, float *vals, int32_t *locs, int64_t ldo, void *stream);
@@ -222,13 +223,13 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words,
  * fused kernel keeps (kprime for RM_MODE_FP8_RESCORE, else ktop). */
 int rm_plan_get(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int32_t *tile_rows,
                 int32_t *splits, int32_t *qgroup);
-/* Rows per fused launch that rm_topk_batch and rm_map_batch use for a batch
- * of M rows (0: one launch).  Without a tuned plan for M's bucket, a batch
+/* Rows per fused launch that rm_topk_batch, rm_map_batch and rm_map_batches
+ * (for the total row count) use for a batch of M rows (0: one launch).  Without a tuned plan for M's bucket, a batch
  * above 64 x CUs rows on a large index (each of the launches' splits
  * sweeping at least 1,024 column tiles) runs as equal launches of at most
  * that many rows, each with rm_plan_get's plan for that many rows: one wave
- * of workgroups with its query tiles resident in the XCDs' L2.  Uneven
- * batches (rm_map_batches) always run as one launch.  No counterpart in the
+ * of workgroups with its query tiles resident in the XCDs' L2.  No
+ * counterpart in the
  * reference, whose engine cuts batches at its own rows_cap
  * (RotorMap/search/engine_fp8.jl:309-337). */
 int rm_launch_rows(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int64_t *rows);

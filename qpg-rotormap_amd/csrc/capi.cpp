@@ -710,8 +710,8 @@ static int encode_queries(rm_ctx *ctx, const uint32_t *words, int64_t n, int k, 
 }
 
 // encode -> fused GEMM + top-k -> merge of n fragments; one_launch keeps the
-// whole batch in one fused launch (the uneven-batch path), else the
-// built-in launch rule applies (topk_batched)
+// whole batch in one fused launch, else the built-in launch rule applies
+// (topk_batched)
 static int map_batch_impl(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, int64_t n,
                           int k, int s, int m, int c, int ktop, int kprime, int mode, float *vals,
                           int32_t *locs, int64_t ldo, float *norms, void *stream, bool one_launch) {
@@ -760,11 +760,12 @@ int rm_map_batches(rm_ctx *ctx, const rm_index *idx, const uint32_t *words, cons
     RM_CHECK(offs[0] == 0, RM_ERR_ARG, "offs[0] must be 0");
     for (int b = 0; b < nb; ++b)
         RM_CHECK(offs[b + 1] >= offs[b], RM_ERR_ARG, "offs must be non-decreasing");
-    // rows are independent: the concatenation is one launch whose (row tile,
-    // split) work list spans every batch -- no per-batch launch, no per-batch
-    // tail; the split count follows the total row count
+    // rows are independent: the concatenation runs as one batch, whose
+    // (row tile, split) work list spans every batch -- no per-batch launch,
+    // no per-batch tail; the plan and the launch rule follow the total row
+    // count
     return map_batch_impl(ctx, idx, words, offs[nb], k, s, m, c, ktop, kprime, mode, vals, locs, ldo,
-                          nullptr, stream, true);
+                          nullptr, stream, false);
 }
 
 int rm_plan_get(rm_ctx *ctx, const rm_index *idx, int64_t M, int mode, int kp, int32_t *tile_rows,

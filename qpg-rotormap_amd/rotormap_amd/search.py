@@ -142,7 +142,9 @@ class TopKEngine:
         return v, l
 
     def map_batches(self, re, words, offs, stream=None):
-        """Uneven batches (row offsets offs, nb+1 entries) in one launch."""
+        """Uneven batches (row offsets offs, nb+1 entries) in one call: one
+        work list over the concatenation, cut into launches as one batch of
+        that many rows would be (launch_rows)."""
         offs = np.ascontiguousarray(offs, dtype=np.int64)
         n = int(offs[-1])
         v, l = self._out(n)

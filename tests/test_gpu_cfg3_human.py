@@ -10,7 +10,8 @@ resident.
   reference's model (golden FP64 encoder, atol 5e-3, RotorMap/search/
   engine_fp8.jl:866-883).
 - Uneven batches: after rm_autotune, one rm_map_batches launch over
-  concatenated batches equals per-batch rm_map_batch, bit for bit.
+  concatenated batches equals per-batch rm_map_batch, bit for bit; so does a
+  concatenation large enough to run as several launches.
 - Serving: a short Poisson trace; every request equals the one-shot rows."""
 import os
 
@@ -137,6 +138,36 @@ def test_human_uneven_batches_after_autotune(torch_gpu, human):
     res = R.count_mapped(lb, mixed.recs, mixed.starts, idx.rec_of_col, idx.starts, K)
     print(f"[cfg4] recall {res['correct']}/{res['total']}")
     assert res["correct"] >= 0.99 * NB
+
+
+def test_human_uneven_batches_cut_into_launches(torch_gpu, human):
+    # 20,480 rows in uneven batches: above one launch's 64 x CUs rows, so the
+    # concatenation runs as the launch rule's launches; every batch must equal
+    # rm_map_batch on it alone (plans differ, results may not)
+    torch = torch_gpu
+    ref, re, idx, exact, mixed, B, dw, _ = human
+    reps = 5
+    n = reps * NB
+    words = np.ascontiguousarray(np.concatenate([mixed.words] * reps))
+    sizes = np.array([3, 16381, 1, n - 16385])
+    eng = R.TopKEngine(B, k=20, kprime=32, mode="fp8_rescore", rows_cap=n)
+    lr = eng.launch_rows(n)
+    print(f"[cfg4] {n} rows as launches of {lr}")
+    assert 0 < lr < n, lr
+    W = (K + 15) // 16
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    dwm = torch.from_numpy(words.view(np.int32).reshape(-1).copy()).cuda()
+    vb, lb = eng.map_batches(re, dwm, offs)
+    parts = [eng.map_words(re, dwm[offs[b] * W:offs[b + 1] * W], int(sizes[b]))
+             for b in range(len(sizes))]
+    torch.cuda.synchronize()
+    vb, lb = vb.cpu().numpy(), lb.cpu().numpy()
+    for b, (v, l) in enumerate(parts):
+        np.testing.assert_array_equal(v.cpu().numpy(), vb[offs[b]:offs[b + 1]], err_msg=f"batch {b}")
+        np.testing.assert_array_equal(l.cpu().numpy(), lb[offs[b]:offs[b + 1]], err_msg=f"batch {b}")
+    # the repeated reads map as they did in the 4,096-read batch
+    for r in range(reps):
+        np.testing.assert_array_equal(lb[r * NB:(r + 1) * NB], lb[:NB], err_msg=f"copy {r}")
 
 
 def test_human_serving_trace(torch_gpu, human):
